@@ -1,0 +1,76 @@
+/*
+ * minfft_ext.h -- extensions of the MI355X minfft drop-in.  None of these
+ * has a counterpart in the reference (aimukhin/minfft); they never change the
+ * 27 prototypes of minfft.h.
+ *
+ *  - error channel: the reference reports errors only as a NULL aux
+ *    (minfft.c:1062-1064); minfft_last_error() says why.
+ *  - device-pointer asynchronous execution on a caller's HIP stream.
+ *  - minfft_*_batch: many transforms of mixed kinds and power-of-two sizes in
+ *    one graph launch (semantics: the single-transform routine applied to
+ *    each entry).
+ */
+#ifndef MINFFT_EXT_H
+#define MINFFT_EXT_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Reason for the last failure on this thread ("" if none).  Exported by both
+   libminfft.so and libminfft_single.so (via libminfft_engine.so). */
+const char *minfft_last_error(void);
+
+/* transform kinds, in the order of minfft.h */
+enum minfft_kind {
+    MINFFT_DFT = 0, MINFFT_INVDFT = 1, MINFFT_REALDFT = 2, MINFFT_INVREALDFT = 3,
+    MINFFT_DCT2 = 4, MINFFT_DST2 = 5, MINFFT_DCT3 = 6, MINFFT_DST3 = 7,
+    MINFFT_DCT4 = 8, MINFFT_DST4 = 9
+};
+
+/* arithmetic of a batch: FP64 throughout; FP32 butterflies with twiddles
+   generated in FP64 and rounded; FP32 data with radix-16 stages computed as
+   split-FP16 (hi+lo) MFMA products accumulated in FP32. */
+enum minfft_prec { MINFFT_FP64 = 0, MINFFT_FP32 = 1, MINFFT_SPLIT_FP16 = 2 };
+
+/* One entry of a batch: `kind` on a d-dimensional (1..3) array of dims n[0..d-1]
+   (C order).  Input and output live at the given byte offsets of the buffers
+   passed to minfft_batch_run(); they must be aligned to the item size
+   (complex or real of the batch precision). */
+typedef struct minfft_batch_entry {
+    int kind;
+    int d;
+    int n[3];
+    long long in_offset;
+    long long out_offset;
+} minfft_batch_entry;
+
+typedef struct minfft_batch_plan minfft_batch_plan;
+
+/* Build a batch plan: validates entries (NULL on error), groups identical
+   shapes, builds the twiddle tables on the device and picks tile schedules. */
+minfft_batch_plan *minfft_mkaux_batch(int count, const minfft_batch_entry *entries, int prec);
+
+/* Run the batch on device buffers `in` / `out` (device pointers) on the HIP
+   stream `stream` (hipStream_t; NULL = the plan's own stream).  The first run
+   with a given (in, out, stream) captures a HIP graph; later runs replay it.
+   Asynchronous: returns 0 once enqueued, -1 on error. */
+int minfft_batch_run(minfft_batch_plan *p, const void *in, void *out, void *stream);
+
+/* Same, but synchronous and accepting host or device pointers; in_bytes and
+   out_bytes give the extents of the two buffers. */
+int minfft_batch(minfft_batch_plan *p, const void *in, size_t in_bytes, void *out, size_t out_bytes);
+
+void minfft_free_batch(minfft_batch_plan *p);
+
+/* Enqueue one transform of an aux on device pointers and a caller's stream
+   (hipStream_t; NULL = the aux's own stream) without waiting.  Returns 0 or -1. */
+int minfft_enqueue(int kind, const void *in, void *out, const void *aux, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MINFFT_EXT_H */

@@ -1,0 +1,116 @@
+"""Build the MI355X minfft libraries in-tree with hipcc (gfx950).
+
+Produces, under minfft_amd/lib/:
+  libminfft_engine.so  kernels (tile/pre/post/twiddle), plans, batch extension
+  libminfft.so         the 27 minfft.h entry points, double precision
+  libminfft_single.so  the same entry points, single precision (MINFFT_SINGLE)
+
+Usage: python -m minfft_amd.build [--force] [-j N]
+"""
+import argparse
+import os
+import subprocess
+import sys
+from concurrent.futures import ThreadPoolExecutor
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+INC = os.path.join(ROOT, "include")
+OBJ = os.path.join(HERE, "_build")
+LIB = os.path.join(HERE, "lib")
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = os.environ.get("MINFFT_ARCH", "gfx950")
+
+COMMON = [
+    f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17",
+    # no packed FP32 math (v_pk_*_f32) in device code
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-I", INC, "-I", CSRC,
+    "-Wall", "-Wno-unused-function", "-Wno-unused-variable", "-Wno-unused-but-set-variable",
+]
+
+HEADERS = ["fft_core.hpp", "kinds.hpp", "tile.hpp", "ktable.hpp", "engine.hpp"]
+
+
+def _newer(target, deps):
+    if not os.path.exists(target):
+        return True
+    t = os.path.getmtime(target)
+    return any(os.path.getmtime(d) > t for d in deps if os.path.exists(d))
+
+
+def _units():
+    """(object, source, extra flags) for every translation unit."""
+    u = []
+    for f64 in (0, 1):
+        for kind in range(10):
+            o = os.path.join(OBJ, f"inst_f{64 if f64 else 32}_k{kind}.o")
+            u.append((o, "inst.hip", [f"-DINST_F64={f64}", f"-DINST_KIND={kind}"], "engine"))
+    u.append((os.path.join(OBJ, "twiddle.o"), "twiddle.hip", [], "engine"))
+    u.append((os.path.join(OBJ, "engine.o"), "engine.cpp", ["-x", "hip"], "engine"))
+    u.append((os.path.join(OBJ, "batch.o"), "batch.cpp", ["-x", "hip"], "engine"))
+    u.append((os.path.join(OBJ, "api64.o"), "api.cpp", ["-x", "hip"], "api64"))
+    u.append((os.path.join(OBJ, "api32.o"), "api.cpp", ["-x", "hip", "-DMINFFT_SINGLE=1"], "api32"))
+    return u
+
+
+def _compile(unit, force):
+    obj, src, extra, _ = unit
+    srcp = os.path.join(CSRC, src)
+    deps = [srcp] + [os.path.join(CSRC, h) for h in HEADERS] + \
+        [os.path.join(INC, "minfft.h"), os.path.join(INC, "minfft_ext.h")]
+    if not force and not _newer(obj, deps):
+        return obj, None
+    cmd = [HIPCC] + COMMON + extra + ["-c", srcp, "-o", obj]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        return obj, f"$ {' '.join(cmd)}\n{r.stdout}\n{r.stderr}"
+    return obj, None
+
+
+def _link(out, objs, extra=()):
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", out] + list(objs) + list(extra)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+
+
+def build(force=False, jobs=None, verbose=True):
+    os.makedirs(OBJ, exist_ok=True)
+    os.makedirs(LIB, exist_ok=True)
+    units = _units()
+    jobs = jobs or min(8, os.cpu_count() or 4)
+    errors = []
+    with ThreadPoolExecutor(max_workers=jobs) as ex:
+        for obj, err in ex.map(lambda u: _compile(u, force), units):
+            if err:
+                errors.append(err)
+    if errors:
+        raise RuntimeError("hipcc failed:\n" + "\n".join(errors))
+    eng = [u[0] for u in units if u[3] == "engine"]
+    eng_so = os.path.join(LIB, "libminfft_engine.so")
+    if force or _newer(eng_so, eng):
+        _link(eng_so, eng)
+    rpath = ["-Wl,-rpath,$ORIGIN", "-L" + LIB, "-lminfft_engine"]
+    for name, tag in (("libminfft.so", "api64"), ("libminfft_single.so", "api32")):
+        objs = [u[0] for u in units if u[3] == tag]
+        so = os.path.join(LIB, name)
+        if force or _newer(so, objs + [eng_so]):
+            _link(so, objs, rpath)
+    if verbose:
+        print(f"built {eng_so}, libminfft.so, libminfft_single.so")
+    return LIB
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--force", action="store_true")
+    ap.add_argument("-j", type=int, default=None)
+    a = ap.parse_args(argv)
+    build(force=a.force, jobs=a.j)
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,294 @@
+// batch.cpp -- minfft_ext.h: error channel, asynchronous enqueue and the
+// batch extension.  A batch is a list of independent entries (kind, dims,
+// byte offsets); its semantics are those of calling the single-transform
+// routine on every entry (minfft.c:287-290 for the complex DFT, 735-738 /
+// 931-934 for DCT-2 / DCT-4, and so on).
+//
+// Execution: entries with identical (kind, dims) form a group; each group is
+// executed as the same per-axis passes as a single transform, with one extra
+// batch level whose base addresses come from device tables of per-entry
+// offsets, so a group of any size costs one launch per axis (two for an axis
+// longer than one tile).  All groups' launches are captured once into a HIP
+// graph per (in, out, stream) and replayed.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <tuple>
+#include <vector>
+
+#include "engine.hpp"
+#include "minfft_ext.h"
+
+using namespace mfx;
+
+struct BatchGroup {
+    int kind = 0;
+    int d = 1;
+    int n[3] = {1, 1, 1};
+    std::vector<long long> ioff, ooff, foff, toff;  // host copies (items)
+    long long* d_ioff = nullptr;
+    long long* d_ooff = nullptr;
+    long long* d_foff = nullptr;
+    long long* d_toff = nullptr;
+    size_t tbytes = 0;     // per-group temp (inverse real DFT, d > 1)
+    size_t fbytes = 0;     // per-group two-step scratch
+    void* tbuf = nullptr;
+};
+
+struct minfft_batch_plan {
+    Prec prec = PREC_F32;
+    int prec_req = 1;
+    std::vector<BatchGroup> groups;
+    Workspace ws;
+    hipStream_t stream = nullptr;
+    // graph cache
+    const void* g_in = nullptr;
+    void* g_out = nullptr;
+    hipStream_t g_st = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+static size_t item_in_bytes(int kind, Prec p) { return (kind_complex_in(kind) ? 2 : 1) * real_size(p); }
+static size_t item_out_bytes(int kind, Prec p) { return (kind_complex_out(kind) ? 2 : 1) * real_size(p); }
+
+static int lg2(long long x) {
+    int l = 0;
+    while ((1LL << l) < x) ++l;
+    return l;
+}
+
+extern "C" {
+
+const char* minfft_last_error(void) { return mfx::last_error(); }
+
+int minfft_enqueue(int kind, const void* in, void* out, const void* aux, void* stream) {
+    clear_error();
+    Plan* p = reinterpret_cast<Plan*>(const_cast<void*>(aux));
+    if (!p || p->magic != 0x6d667870u) {
+        set_error("invalid aux data");
+        return -1;
+    }
+    if (kind < 0 || kind >= K_COUNT) {
+        set_error("invalid kind");
+        return -1;
+    }
+    hipStream_t st = stream ? reinterpret_cast<hipStream_t>(stream) : p->stream;
+    return enqueue(*p, kind, in, out, st);
+}
+
+void minfft_free_batch(minfft_batch_plan* p) {
+    if (!p) return;
+    if (p->exec) (void)hipGraphExecDestroy(p->exec);
+    if (p->graph) (void)hipGraphDestroy(p->graph);
+    for (auto& g : p->groups) {
+        if (g.d_ioff) (void)hipFree(g.d_ioff);
+        if (g.d_ooff) (void)hipFree(g.d_ooff);
+        if (g.d_foff) (void)hipFree(g.d_foff);
+        if (g.d_toff) (void)hipFree(g.d_toff);
+        if (g.tbuf) (void)hipFree(g.tbuf);
+    }
+    free_workspace(p->ws);
+    if (p->stream) (void)hipStreamDestroy(p->stream);
+    delete p;
+}
+
+minfft_batch_plan* minfft_mkaux_batch(int count, const minfft_batch_entry* e, int prec) {
+    clear_error();
+    if (count <= 0 || !e) {
+        set_error("empty batch");
+        return nullptr;
+    }
+    if (prec != MINFFT_FP64 && prec != MINFFT_FP32) {
+        set_error("precision not supported by this batch path");
+        return nullptr;
+    }
+    if (!ensure_device()) return nullptr;
+    minfft_batch_plan* p = new minfft_batch_plan();
+    p->prec = prec == MINFFT_FP64 ? PREC_F64 : PREC_F32;
+    p->prec_req = prec;
+    auto fail = [&](const char* msg) -> minfft_batch_plan* {
+        if (msg) set_error(msg);
+        minfft_free_batch(p);
+        return nullptr;
+    };
+    std::map<std::tuple<int, int, int, int, int>, int> idx;
+    int twlog = 1;
+    size_t fs_max = 0;
+    for (int i = 0; i < count; ++i) {
+        const minfft_batch_entry& x = e[i];
+        if (x.kind < 0 || x.kind >= K_COUNT) return fail("invalid entry kind");
+        if (x.d < 1 || x.d > 3) return fail("entry dimensions must be 1..3");
+        int n[3] = {1, 1, 1};
+        long long P = 1;
+        for (int k = 0; k < x.d; ++k) {
+            if (x.n[k] <= 0 || (x.n[k] & (x.n[k] - 1))) return fail("entry lengths must be powers of two");
+            n[k] = x.n[k];
+            P *= n[k];
+        }
+        const size_t ib = item_in_bytes(x.kind, p->prec), ob = item_out_bytes(x.kind, p->prec);
+        if (x.in_offset < 0 || x.out_offset < 0 || x.in_offset % (long long)ib || x.out_offset % (long long)ob)
+            return fail("entry offsets must be non-negative multiples of the item size");
+        auto key = std::make_tuple(x.kind, x.d, n[0], n[1], n[2]);
+        auto it = idx.find(key);
+        int gi;
+        if (it == idx.end()) {
+            gi = (int)p->groups.size();
+            idx[key] = gi;
+            BatchGroup g;
+            g.kind = x.kind;
+            g.d = x.d;
+            std::memcpy(g.n, n, sizeof(n));
+            p->groups.push_back(g);
+        } else {
+            gi = it->second;
+        }
+        BatchGroup& g = p->groups[gi];
+        g.ioff.push_back(x.in_offset / (long long)ib);
+        g.ooff.push_back(x.out_offset / (long long)ob);
+        (void)P;
+    }
+    p->ws = Workspace();
+    if (hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess) return fail("hipStreamCreate failed");
+    const size_t cs = 2 * real_size(p->prec);
+    for (auto& g : p->groups) {
+        const unsigned cnt = (unsigned)g.ioff.size();
+        long long P = 1;
+        for (int k = 0; k < g.d; ++k) P *= g.n[k];
+        const long long Nd = g.n[g.d - 1], H = Nd / 2 + 1, rows = P / Nd;
+        const bool realfam = g.kind == K_REALDFT || g.kind == K_INVREALDFT;
+        // per-entry complex extent of the arrays the complex passes touch
+        const long long cext = realfam ? rows * H : P;
+        size_t fs = 0;
+        for (int a = 0; a < g.d; ++a) {
+            int kind = g.kind;
+            if (realfam && a < g.d - 1) kind = K_DFT;
+            twlog = std::max(twlog, needed_tw_log(kind, lg2(g.n[a])));
+            const long long others = (realfam && a < g.d - 1) ? rows / g.n[a] * H : P / g.n[a];
+            fs = std::max(fs, axis_scratch_bytes(p->prec, kind, lg2(g.n[a]), (unsigned long long)others * cnt,
+                                                 (unsigned long long)cext * cnt));
+        }
+        g.fbytes = fs;
+        fs_max = std::max(fs_max, fs);
+        g.foff.resize(cnt);
+        for (unsigned i = 0; i < cnt; ++i) g.foff[i] = (long long)i * cext;
+        if (g.kind == K_INVREALDFT && g.d > 1) {
+            g.tbytes = (size_t)cnt * (size_t)(rows * H) * cs;
+            g.toff.resize(cnt);
+            for (unsigned i = 0; i < cnt; ++i) g.toff[i] = (long long)i * rows * H;
+            if (hipMalloc(&g.tbuf, g.tbytes) != hipSuccess) return fail("hipMalloc batch temp failed");
+        }
+        auto up = [&](std::vector<long long>& h, long long*& dptr) -> bool {
+            if (h.empty()) return true;
+            if (hipMalloc(&dptr, h.size() * sizeof(long long)) != hipSuccess) return false;
+            return hipMemcpy(dptr, h.data(), h.size() * sizeof(long long), hipMemcpyHostToDevice) == hipSuccess;
+        };
+        if (!up(g.ioff, g.d_ioff) || !up(g.ooff, g.d_ooff) || !up(g.foff, g.d_foff) || !up(g.toff, g.d_toff))
+            return fail("uploading batch tables failed");
+    }
+    // largest groups first
+    std::stable_sort(p->groups.begin(), p->groups.end(), [](const BatchGroup& a, const BatchGroup& b) {
+        long long pa = 1, pb = 1;
+        for (int k = 0; k < a.d; ++k) pa *= a.n[k];
+        for (int k = 0; k < b.d; ++k) pb *= b.n[k];
+        return pa * (long long)a.ioff.size() > pb * (long long)b.ioff.size();
+    });
+    if (alloc_twiddles(p->ws, p->prec, twlog, p->stream)) return fail(nullptr);
+    if (fs_max) {
+        if (hipMalloc(&p->ws.fbuf, fs_max) != hipSuccess) return fail("hipMalloc batch scratch failed");
+        if (hipMalloc(&p->ws.gbuf, fs_max) != hipSuccess) return fail("hipMalloc batch scratch failed");
+        p->ws.fs_bytes = fs_max;
+    }
+    if (hipStreamSynchronize(p->stream) != hipSuccess) return fail("twiddle generation failed");
+    return p;
+}
+
+static int enqueue_batch(minfft_batch_plan* p, const void* in, void* out, hipStream_t st) {
+    for (auto& g : p->groups) {
+        int rc = run_entries(p->ws, p->prec, g.kind, g.d, g.n, in, out, (unsigned)g.ioff.size(), g.d_ioff,
+                             g.d_ooff, g.d_foff, g.tbuf, g.d_toff, st);
+        if (rc) return rc;
+    }
+    return 0;
+}
+
+int minfft_batch_run(minfft_batch_plan* p, const void* in, void* out, void* stream) {
+    clear_error();
+    if (!p) {
+        set_error("invalid batch plan");
+        return -1;
+    }
+    hipStream_t st = stream ? reinterpret_cast<hipStream_t>(stream) : p->stream;
+    if (!(p->exec && p->g_in == in && p->g_out == out && p->g_st == st)) {
+        if (p->exec) (void)hipGraphExecDestroy(p->exec);
+        if (p->graph) (void)hipGraphDestroy(p->graph);
+        p->exec = nullptr;
+        p->graph = nullptr;
+        if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+            set_error("hipStreamBeginCapture failed");
+            return -1;
+        }
+        int rc = enqueue_batch(p, in, out, st);
+        hipGraph_t g = nullptr;
+        hipError_t ec = hipStreamEndCapture(st, &g);
+        if (rc || ec != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            if (!rc) set_error("hipStreamEndCapture failed");
+            return -1;
+        }
+        if (hipGraphInstantiate(&p->exec, g, nullptr, nullptr, 0) != hipSuccess) {
+            (void)hipGraphDestroy(g);
+            set_error("hipGraphInstantiate failed");
+            return -1;
+        }
+        p->graph = g;
+        p->g_in = in;
+        p->g_out = out;
+        p->g_st = st;
+    }
+    if (hipGraphLaunch(p->exec, st) != hipSuccess) {
+        set_error("hipGraphLaunch failed");
+        return -1;
+    }
+    return 0;
+}
+
+int minfft_batch(minfft_batch_plan* p, const void* in, size_t in_bytes, void* out, size_t out_bytes) {
+    clear_error();
+    if (!p) {
+        set_error("invalid batch plan");
+        return -1;
+    }
+    const bool din = is_device_pointer(in), dout = is_device_pointer(out);
+    if (din && dout) {
+        if (minfft_batch_run(p, in, out, nullptr)) return -1;
+        if (hipStreamSynchronize(p->stream) != hipSuccess) {
+            set_error("batch execution failed");
+            return -1;
+        }
+        return 0;
+    }
+    void* di = nullptr;
+    void* dout_p = nullptr;
+    if (hipMalloc(&di, std::max<size_t>(in_bytes, 1)) != hipSuccess ||
+        hipMalloc(&dout_p, std::max<size_t>(out_bytes, 1)) != hipSuccess) {
+        if (di) (void)hipFree(di);
+        set_error("hipMalloc staging failed");
+        return -1;
+    }
+    int rc = 0;
+    if (hipMemcpyAsync(di, in, in_bytes, hipMemcpyDefault, p->stream) != hipSuccess) rc = -1;
+    // regions of `out` not covered by any entry keep their contents
+    if (!rc && hipMemcpyAsync(dout_p, out, out_bytes, hipMemcpyDefault, p->stream) != hipSuccess) rc = -1;
+    if (!rc) rc = enqueue_batch(p, di, dout_p, p->stream);
+    if (!rc && hipMemcpyAsync(out, dout_p, out_bytes, hipMemcpyDefault, p->stream) != hipSuccess) rc = -1;
+    if (hipStreamSynchronize(p->stream) != hipSuccess) rc = -1;
+    (void)hipFree(di);
+    (void)hipFree(dout_p);
+    if (rc && !*last_error()) set_error("batch execution failed");
+    return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,681 @@
+// engine.cpp -- plans, pass scheduling and launches for the transform engine.
+//
+// A d-dimensional transform is executed as one batched 1D pass per axis,
+// last axis first, exactly the order in which the reference's mkcx/mkrx
+// recursion applies its 1D routine (minfft.c:33-72); the real DFT runs its
+// real pass along the last axis first and then complex passes over the
+// leading axes (minfft.c:582-597), its inverse the other way round
+// (minfft.c:662-687).  Each pass is one launch of the fused tile kernel when
+// the axis length fits one work-group (2^14 FP32 / 2^13 FP64 points on a
+// 160 KiB-LDS CU), otherwise a two-step (N = N1*N2) decomposition through a
+// scratch buffer.
+#include "engine.hpp"
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+
+#include "ktable.hpp"
+#include "tile.hpp"
+
+namespace mfx {
+
+hipError_t launch_twiddles(double2* tw64, float2* tw32, long long total, hipStream_t st);
+
+// ------------------------------------------------------------------ errors
+static thread_local std::string g_err;
+void set_error(const std::string& msg) { g_err = msg; }
+const char* last_error() { return g_err.c_str(); }
+void clear_error() { g_err.clear(); }
+
+#define HIPCHK(expr, what)                                                           \
+    do {                                                                             \
+        hipError_t e_ = (expr);                                                      \
+        if (e_ != hipSuccess) {                                                      \
+            set_error(std::string(what) + ": " + hipGetErrorString(e_));             \
+            return -1;                                                               \
+        }                                                                            \
+    } while (0)
+
+int family_of(int kind) {
+    switch (kind) {
+        case K_DFT: case K_INVDFT: return FAM_DFT;
+        case K_REALDFT: case K_INVREALDFT: return FAM_REALDFT;
+        case K_DCT2: case K_DST2: case K_DCT3: case K_DST3: return FAM_T2T3;
+        default: return FAM_T4;
+    }
+}
+
+size_t real_size(Prec p) { return p == PREC_F64 ? 8 : 4; }
+
+static int ilog2i(long long x) {
+    int l = 0;
+    while ((1LL << l) < x) ++l;
+    return l;
+}
+
+// --------------------------------------------------------------- registry
+#define DECL(k) void register_f32_k##k(KTable&); void register_f64_k##k(KTable&);
+DECL(0) DECL(1) DECL(2) DECL(3) DECL(4) DECL(5) DECL(6) DECL(7) DECL(8) DECL(9)
+#undef DECL
+
+void register_f32(KTable& t) {
+    register_f32_k0(t); register_f32_k1(t); register_f32_k2(t); register_f32_k3(t); register_f32_k4(t);
+    register_f32_k5(t); register_f32_k6(t); register_f32_k7(t); register_f32_k8(t); register_f32_k9(t);
+}
+void register_f64(KTable& t) {
+    register_f64_k0(t); register_f64_k1(t); register_f64_k2(t); register_f64_k3(t); register_f64_k4(t);
+    register_f64_k5(t); register_f64_k6(t); register_f64_k7(t); register_f64_k8(t); register_f64_k9(t);
+}
+
+static KTable g_kt[2];
+static std::once_flag g_kt_once;
+static const KTable& ktable(Prec p) {
+    std::call_once(g_kt_once, [] {
+        std::memset(g_kt, 0, sizeof(g_kt));
+        register_f64(g_kt[PREC_F64]);
+        register_f32(g_kt[PREC_F32]);
+    });
+    return g_kt[p];
+}
+
+// ----------------------------------------------------------------- device
+static Device g_dev;
+static std::mutex g_dev_mu;
+
+Device& device() { return g_dev; }
+
+bool ensure_device() {
+    std::lock_guard<std::mutex> lk(g_dev_mu);
+    int cur = -1;
+    if (hipGetDevice(&cur) != hipSuccess) {
+        set_error("no HIP device available");
+        return false;
+    }
+    if (g_dev.id == cur) return true;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, cur) != hipSuccess) {
+        set_error("hipGetDeviceProperties failed");
+        return false;
+    }
+    g_dev.id = cur;
+    g_dev.ncu = prop.multiProcessorCount;
+    int lds = (int)prop.sharedMemPerBlock;
+    int optin = 0;
+    if (hipDeviceGetAttribute(&optin, hipDeviceAttributeSharedMemPerBlockOptin, cur) == hipSuccess &&
+        optin > lds)
+        lds = optin;
+    g_dev.lds_max = lds;
+    g_dev.nxcd = 8;
+    const KTable& k64 = ktable(PREC_F64);
+    const KTable& k32 = ktable(PREC_F32);
+    for (int p = 0; p < 2; ++p) {
+        const size_t cs = p == PREC_F64 ? 16 : 8;
+        const KTable& kt = p == PREC_F64 ? k64 : k32;
+        int m = 0;
+        for (int l = 0; l <= kt.maxlog; ++l)
+            if ((size_t)slot_len(1 << l) * cs <= (size_t)lds) m = l;
+        g_dev.maxlog[p] = m;
+    }
+    return true;
+}
+
+bool is_device_pointer(const void* p) {
+    if (!p) return false;
+    hipPointerAttribute_t a;
+    hipError_t e = hipPointerGetAttributes(&a, p);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged ||
+           a.type == hipMemoryTypeUnified;
+}
+
+// -------------------------------------------------------------- launching
+static std::mutex g_attr_mu;
+
+static int launch_fn(const void* fn, unsigned grid, unsigned block, void** args, size_t lds,
+                     hipStream_t st) {
+    if (lds > 65536) {
+        std::lock_guard<std::mutex> lk(g_attr_mu);
+        (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    }
+    HIPCHK(hipLaunchKernel(fn, dim3(grid), dim3(block), args, lds, st), "kernel launch");
+    return 0;
+}
+
+// Place a pass's batch levels into TileArgs.  TileArgs levels are
+// right-aligned (level 3 fastest); an entry-offset table always belongs to
+// TileArgs level 0, so a Levels whose slowest level carries a table keeps
+// that level at 0 and right-aligns the rest.  *fs_level (an index into lv,
+// or -1) is translated the same way.
+static void fill_levels(TileArgs& A, const Levels& lv, int* fs_level = nullptr) {
+    for (int k = 0; k < 4; ++k) {
+        A.B[k] = 1;
+        A.in_s[k] = A.out_s[k] = 0;
+    }
+    const bool tab = (lv.ioff || lv.ooff) && lv.n > 0;
+    const int off = 4 - lv.n;
+    for (int k = 0; k < lv.n; ++k) {
+        const int dst = (tab && k == 0) ? 0 : off + k;
+        A.B[dst] = lv.B[k];
+        A.in_s[dst] = lv.is[k];
+        A.out_s[dst] = lv.os[k];
+    }
+    if (fs_level && *fs_level >= 0) *fs_level = (tab && *fs_level == 0) ? 0 : off + *fs_level;
+    A.in_off = tab ? lv.ioff : nullptr;
+    A.out_off = tab ? lv.ooff : nullptr;
+}
+
+// One launch of the fused tile kernel.
+static int launch_tile(Prec prec, int kind, int logL, const void* tw, const void* in, long long in_e,
+                       void* out, long long out_e, const Levels& lv, int fs_level, int fs_logN,
+                       hipStream_t st) {
+    const KTable& kt = ktable(prec);
+    Device& D = device();
+    if (logL > D.maxlog[prec]) {
+        set_error("internal: tile length beyond the single-tile limit");
+        return -1;
+    }
+    const void* fn = kt.tile[kind][logL];
+    const int E = kt.tile_E[logL];
+    const int L = 1 << logL;
+    const int TPT = L / E;
+    TileArgs A;
+    std::memset(&A, 0, sizeof(A));
+    A.in = in;
+    A.out = out;
+    A.tw = tw;
+    A.in_e = in_e;
+    A.out_e = out_e;
+    fill_levels(A, lv, &fs_level);
+    const unsigned long long ntr = lv.count();
+    if (ntr == 0) return 0;
+    if (ntr > 0xffffffffull) {
+        set_error("batch too large");
+        return -1;
+    }
+    A.ntrans = (unsigned)ntr;
+    A.fs_level = fs_level < 0 ? 0 : fs_level;
+    A.fs_logN = fs_logN;
+
+    int f = -1;
+    for (int k = 3; k >= 0; --k)
+        if (A.B[k] > 1) { f = k; break; }
+    const bool tab_f = (f == 0) && (A.in_off || A.out_off);
+    const bool in_row = in_e == 1, out_row = out_e == 1;
+    const bool in_cf = f >= 0 && !tab_f && A.in_s[f] == 1;
+    const bool out_cf = f >= 0 && !tab_f && A.out_s[f] == 1;
+    const bool cplx = (kind == K_DFT || kind == K_INVDFT);
+    if (cplx) {
+        if (in_row && out_row) { A.cfast = 0; A.in_mode = IO_DIRECT; A.out_mode = IO_DIRECT; }
+        else if (in_cf && out_cf) { A.cfast = 1; A.in_mode = IO_DIRECT; A.out_mode = IO_DIRECT; }
+        else if (out_cf) { A.cfast = 1; A.out_mode = IO_DIRECT; A.in_mode = in_row ? IO_STAGE_ROW : IO_DIRECT; }
+        else if (out_row) { A.cfast = 0; A.out_mode = IO_DIRECT; A.in_mode = in_cf ? IO_STAGE_CF : IO_DIRECT; }
+        else { A.cfast = in_cf ? 1 : 0; A.in_mode = IO_DIRECT; A.out_mode = IO_DIRECT; }
+    } else {
+        A.in_mode = in_row ? IO_STAGE_ROW : (in_cf ? IO_STAGE_CF : IO_STAGE_ROW);
+        A.out_mode = out_row ? IO_STAGE_ROW : (out_cf ? IO_STAGE_CF : IO_STAGE_ROW);
+        A.cfast = (A.in_mode == IO_STAGE_CF) ? 1 : 0;
+    }
+    const bool wants_cf = A.cfast || A.in_mode == IO_STAGE_CF || A.out_mode == IO_STAGE_CF;
+    const size_t cs = prec == PREC_F64 ? 16 : 8;
+    const int sc = slot_len(L);
+    const int maxT = prec == PREC_F64 ? 512 : 1024;  // max_threads<T>() of tile.hpp
+    int C;
+    if (wants_cf) {
+        // several columns per tile so each row segment spans >= 64 B
+        C = (int)(128 / cs);
+        while (C > 1 && C * TPT > maxT) C >>= 1;
+    } else {
+        C = std::max(1, 256 / TPT);
+    }
+    if ((unsigned long long)C > ntr) C = (int)ntr;
+    size_t budget = (size_t)D.lds_max;
+    while (C > 1 && (size_t)C * sc * cs > budget) C >>= 1;
+    if ((size_t)C * sc * cs > budget) {
+        set_error("internal: LDS budget exceeded");
+        return -1;
+    }
+    A.C = C;
+    A.sc = sc;
+    const unsigned long long tiles = (ntr + C - 1) / C;
+    A.ntiles = (unsigned)tiles;
+    A.xcd = (tiles >= 2ull * D.nxcd) ? D.nxcd : 1;
+    void* args[] = {&A};
+    return launch_fn(fn, (unsigned)tiles, (unsigned)(C * TPT), args, (size_t)C * sc * cs, st);
+}
+
+static int launch_elementwise(const void* fn, TileArgs& A, int L, long long total, hipStream_t st) {
+    if (total <= 0) return 0;
+    const unsigned bs = 256;
+    const unsigned long long grid = ((unsigned long long)total + bs - 1) / bs;
+    void* args[] = {&A, &L, &total};
+    return launch_fn(fn, (unsigned)grid, bs, args, 0, st);
+}
+
+int needed_tw_log(int kind, int logN) {
+    if (kind == K_DFT || kind == K_INVDFT) return std::max(logN, 1);
+    return std::max(logN + 3, 1);
+}
+
+size_t axis_scratch_bytes(Prec prec, int kind, int logN, unsigned long long ntrans,
+                          unsigned long long in_extent) {
+    if (!ensure_device()) return 0;
+    const int maxlog = device().maxlog[prec];
+    const size_t cs = prec == PREC_F64 ? 16 : 8;
+    if (kind == K_DFT || kind == K_INVDFT) {
+        if (logN <= maxlog) return 0;
+        return (size_t)in_extent * cs;
+    }
+    if (logN == 0 || logN - 1 <= maxlog) return 0;
+    return (size_t)ntrans * ((size_t)1 << (logN - 1)) * cs;
+}
+
+// complex pass along an axis of element stride e (same on both sides)
+static int c2c_axis(const Workspace& ws, Prec prec, int kind, int logL, const void* in, long long in_e,
+                    void* out, long long out_e, const Levels& lv, hipStream_t st) {
+    const int maxlog = device().maxlog[prec];
+    if (logL <= maxlog)
+        return launch_tile(prec, kind, logL, ws.tw, in, in_e, out, out_e, lv, -1, 0, st);
+    if (in_e != out_e) {
+        set_error("internal: two-step pass needs equal strides");
+        return -1;
+    }
+    const long long e = in_e;
+    const int l1 = logL / 2, l2 = logL - l1;
+    if (l2 > maxlog) {
+        set_error("transform length exceeds the two-step limit");
+        return -1;
+    }
+    if (!ws.fbuf || !ws.tw) {
+        set_error("internal: missing two-step scratch");
+        return -1;
+    }
+    const long long N1 = 1LL << l1, N2 = 1LL << l2;
+    if (lv.n >= 4) {
+        set_error("internal: too many batch levels");
+        return -1;
+    }
+    // step 1: length-N1 columns (stride N2*e), scratch mirrors the input
+    // index space; twiddle W_N^{n2*k1}
+    Levels s1, s2;
+    const long long* fo = lv.foff ? lv.foff : lv.ioff;
+    s1.ioff = lv.ioff;
+    s1.ooff = fo;
+    s2.ioff = fo;
+    s2.ooff = lv.ooff;
+    int fs_pos;
+    if (e == 1) {
+        for (int k = 0; k < lv.n; ++k) {
+            s1.push(lv.B[k], lv.is[k], lv.is[k]);
+            s2.push(lv.B[k], lv.is[k], lv.os[k]);
+        }
+        fs_pos = s1.n;
+        s1.push((unsigned)N2, 1, 1);
+        s2.push((unsigned)N1, N2, 1);
+    } else {
+        for (int k = 0; k + 1 < lv.n; ++k) {
+            s1.push(lv.B[k], lv.is[k], lv.is[k]);
+            s2.push(lv.B[k], lv.is[k], lv.os[k]);
+        }
+        fs_pos = s1.n;
+        s1.push((unsigned)N2, e, e);
+        s2.push((unsigned)N1, N2 * e, e);
+        if (lv.n >= 1) {
+            s1.push(lv.B[lv.n - 1], lv.is[lv.n - 1], lv.is[lv.n - 1]);
+            s2.push(lv.B[lv.n - 1], lv.is[lv.n - 1], lv.os[lv.n - 1]);
+        }
+    }
+    if ((lv.ioff || lv.ooff) && fs_pos == 0) {
+        set_error("internal: offset table level clash");
+        return -1;
+    }
+    int rc = launch_tile(prec, kind, l1, ws.tw, in, N2 * e, ws.fbuf, N2 * e, s1, fs_pos, logL, st);
+    if (rc) return rc;
+    return launch_tile(prec, kind, l2, ws.tw, ws.fbuf, e, out, N1 * e, s2, -1, 0, st);
+}
+
+int axis_pass(const Workspace& ws, Prec prec, int kind, int logN, const void* in, long long in_e,
+              void* out, long long out_e, const Levels& lv, hipStream_t st) {
+    if (!ensure_device()) return -1;
+    const KTable& kt = ktable(prec);
+    const int maxlog = device().maxlog[prec];
+    if (kind == K_DFT || kind == K_INVDFT) return c2c_axis(ws, prec, kind, logN, in, in_e, out, out_e, lv, st);
+    if (logN == 0) {
+        TileArgs A;
+        std::memset(&A, 0, sizeof(A));
+        fill_levels(A, lv);
+        A.in = in;
+        A.out = out;
+        A.in_e = in_e;
+        A.out_e = out_e;
+        const unsigned long long ntr = lv.count();
+        A.ntrans = (unsigned)ntr;
+        const unsigned bs = 256;
+        void* args[] = {&A};
+        return launch_fn(kt.n1[kind], (unsigned)((ntr + bs - 1) / bs), bs, args, 0, st);
+    }
+    const int logL = logN - 1;
+    if (logL <= maxlog)
+        return launch_tile(prec, kind, logL, ws.tw, in, in_e, out, out_e, lv, -1, 0, st);
+    // long real transform: pre-kernel -> two-step complex FFT -> post-kernel
+    if (!ws.gbuf || !ws.fbuf) {
+        set_error("internal: missing two-step scratch");
+        return -1;
+    }
+    if (lv.ioff || lv.ooff) {
+        set_error("long real transforms are not supported in batches");
+        return -1;
+    }
+    const int L = 1 << logL;
+    const unsigned long long ntr = lv.count();
+    TileArgs A;
+    std::memset(&A, 0, sizeof(A));
+    fill_levels(A, lv);
+    A.ntrans = (unsigned)ntr;
+    A.tw = ws.tw;
+    A.in = in;
+    A.in_e = in_e;
+    A.out = ws.gbuf;
+    int rc = launch_elementwise(kt.pre[kind], A, L, (long long)ntr * L, st);
+    if (rc) return rc;
+    Levels g;
+    if (ntr > 1) g.push((unsigned)ntr, L, L);
+    Workspace w2 = ws;
+    rc = c2c_axis(w2, prec, kind_inverse_fft(kind) ? K_INVDFT : K_DFT, logL, ws.gbuf, 1, ws.gbuf, 1, g, st);
+    if (rc) return rc;
+    TileArgs B;
+    std::memset(&B, 0, sizeof(B));
+    fill_levels(B, lv);
+    B.ntrans = (unsigned)ntr;
+    B.tw = ws.tw;
+    B.in = ws.gbuf;
+    B.out = out;
+    B.out_e = out_e;
+    return launch_elementwise(kt.post[kind], B, L, (long long)ntr * kind_nout(kind, L), st);
+}
+
+// ------------------------------------------------------------------ plans
+int alloc_twiddles(Workspace& ws, Prec prec, int log_m, hipStream_t st) {
+    const long long total = 2LL << log_m;
+    HIPCHK(hipMalloc(&ws.tw64, (size_t)total * sizeof(double2)), "hipMalloc twiddles");
+    if (prec == PREC_F32) {
+        HIPCHK(hipMalloc(&ws.tw, (size_t)total * sizeof(float2)), "hipMalloc twiddles");
+    } else {
+        ws.tw = ws.tw64;
+    }
+    ws.tw_log = log_m;
+    HIPCHK(launch_twiddles((double2*)ws.tw64, prec == PREC_F32 ? (float2*)ws.tw : nullptr, total, st),
+           "twiddle kernel");
+    return 0;
+}
+
+void free_workspace(Workspace& ws) {
+    if (ws.tw && ws.tw != ws.tw64) (void)hipFree(ws.tw);
+    if (ws.tw64) (void)hipFree(ws.tw64);
+    if (ws.fbuf) (void)hipFree(ws.fbuf);
+    if (ws.gbuf) (void)hipFree(ws.gbuf);
+    ws = Workspace();
+}
+
+static long long prod_range(const std::vector<int>& v, int a, int b) {
+    long long p = 1;
+    for (int i = a; i < b; ++i) p *= v[i];
+    return p;
+}
+
+// kinds used by a family (for sizing)
+static void family_kinds(Family f, int* k, int& n) {
+    n = 0;
+    switch (f) {
+        case FAM_DFT: k[n++] = K_DFT; break;
+        case FAM_REALDFT: k[n++] = K_REALDFT; break;
+        case FAM_T2T3: k[n++] = K_DCT2; break;
+        default: k[n++] = K_DCT4; break;
+    }
+}
+
+void free_plan(Plan* p) {
+    if (!p) return;
+    if (p->stream && p->own_stream) (void)hipStreamDestroy(p->stream);
+    free_workspace(p->ws);
+    if (p->tbuf) (void)hipFree(p->tbuf);
+    if (p->stage) (void)hipFree(p->stage);
+    p->magic = 0;
+    delete p;
+}
+
+Plan* make_plan(Family fam, Prec prec, int d, const int* Ns) {
+    clear_error();
+    if (d < 1 || !Ns) {
+        set_error("invalid number of dimensions");
+        return nullptr;
+    }
+    long long P = 1;
+    for (int i = 0; i < d; ++i) {
+        const int N = Ns[i];
+        if (N <= 0 || (N & (N - 1))) {
+            set_error("transform lengths must be positive powers of two");
+            return nullptr;
+        }
+        P *= N;
+        if (P > (1LL << 40)) {
+            set_error("transform too large");
+            return nullptr;
+        }
+    }
+    if (!ensure_device()) return nullptr;
+    Plan* p = new Plan();
+    p->fam = fam;
+    p->prec = prec;
+    p->d = d;
+    p->Ns.assign(Ns, Ns + d);
+    p->P = P;
+    p->dev = device().id;
+    auto fail = [&](const char* what) -> Plan* {
+        if (what && !*last_error()) set_error(what);
+        free_plan(p);
+        return nullptr;
+    };
+    if (hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess)
+        return fail("hipStreamCreate failed");
+    const size_t rs = real_size(prec), cs = 2 * rs;
+    // twiddle range and scratch over all axis passes of the family
+    int kinds[2], nk;
+    family_kinds(fam, kinds, nk);
+    int twlog = 1;
+    size_t fs = 0;
+    const int maxlog = device().maxlog[prec];
+    const long long Nd = Ns[d - 1];
+    const long long H = Nd / 2 + 1;
+    const long long rows = P / Nd;
+    for (int a = 0; a < d; ++a) {
+        const int logN = ilog2i(Ns[a]);
+        int kind = kinds[0];
+        if (fam == FAM_REALDFT && a < d - 1) kind = K_DFT;
+        twlog = std::max(twlog, needed_tw_log(kind, logN));
+        const long long others = (fam == FAM_REALDFT && a < d - 1) ? rows / Ns[a] * H : P / Ns[a];
+        const long long extent = (fam == FAM_REALDFT && a < d - 1) ? rows * H : P;
+        fs = std::max(fs, axis_scratch_bytes(prec, kind, logN, (unsigned long long)others,
+                                             (unsigned long long)extent));
+        if (kind == K_DFT && logN > maxlog) {
+            const int l2 = logN - logN / 2;
+            if (l2 > maxlog) return fail("transform length exceeds the two-step limit");
+        }
+        if (kind != K_DFT && logN >= 1 && logN - 1 > maxlog) {
+            const int lL = logN - 1, l2 = lL - lL / 2;
+            if (l2 > maxlog) return fail("transform length exceeds the two-step limit");
+        }
+    }
+    (void)cs;
+    if (alloc_twiddles(p->ws, prec, twlog, p->stream)) return fail(nullptr);
+    if (fs) {
+        if (hipMalloc(&p->ws.fbuf, fs) != hipSuccess) return fail("hipMalloc scratch failed");
+        if (hipMalloc(&p->ws.gbuf, fs) != hipSuccess) return fail("hipMalloc scratch failed");
+        p->ws.fs_bytes = fs;
+    }
+    if (fam == FAM_REALDFT && d > 1) {
+        p->tbuf_bytes = (size_t)(rows * H) * 2 * rs;
+        if (hipMalloc(&p->tbuf, p->tbuf_bytes) != hipSuccess) return fail("hipMalloc scratch failed");
+    }
+    if (hipStreamSynchronize(p->stream) != hipSuccess) return fail("twiddle generation failed");
+    return p;
+}
+
+size_t in_bytes(const Plan& p, int kind) {
+    const size_t rs = real_size(p.prec);
+    const long long Nd = p.Ns[p.d - 1];
+    if (kind == K_DFT || kind == K_INVDFT) return (size_t)p.P * 2 * rs;
+    if (kind == K_INVREALDFT) return (size_t)(p.P / Nd * (Nd / 2 + 1)) * 2 * rs;
+    return (size_t)p.P * rs;
+}
+
+size_t out_bytes(const Plan& p, int kind) {
+    const size_t rs = real_size(p.prec);
+    const long long Nd = p.Ns[p.d - 1];
+    if (kind == K_DFT || kind == K_INVDFT) return (size_t)p.P * 2 * rs;
+    if (kind == K_REALDFT) return (size_t)(p.P / Nd * (Nd / 2 + 1)) * 2 * rs;
+    return (size_t)p.P * rs;
+}
+
+static bool overlap(const void* a, size_t na, const void* b, size_t nb) {
+    const char* x = (const char*)a;
+    const char* y = (const char*)b;
+    return x < y + nb && y < x + na;
+}
+
+// levels of the pass along axis a of an array with dims D (item units)
+static Levels axis_levels(const std::vector<long long>& Dm, int a, long long& S) {
+    const int d = (int)Dm.size();
+    S = 1;
+    for (int i = a + 1; i < d; ++i) S *= Dm[i];
+    long long outer = 1;
+    for (int i = 0; i < a; ++i) outer *= Dm[i];
+    Levels lv;
+    if (outer > 1) lv.push((unsigned)outer, Dm[a] * S, Dm[a] * S);
+    if (S > 1) lv.push((unsigned)S, 1, 1);
+    return lv;
+}
+
+int run_entries(const Workspace& ws, Prec pr, int kind, int d, const int* Ns, const void* in, void* out,
+                unsigned count, const long long* ioff, const long long* ooff, const long long* foff,
+                void* tbuf, const long long* toff, hipStream_t st) {
+    std::vector<long long> Dm(Ns, Ns + d);
+    long long P = 1;
+    for (int i = 0; i < d; ++i) P *= Dm[i];
+    const bool tab = ioff != nullptr;
+    // batch levels of one pass: the entry level (tables) then the axis levels
+    auto mk = [&](const Levels& ax, const long long* it, const long long* ot) {
+        Levels lv;
+        if (tab) {
+            lv.push(count, 0, 0);
+            lv.ioff = it;
+            lv.ooff = ot;
+            lv.foff = foff;
+        }
+        for (int k = 0; k < ax.n; ++k) lv.push(ax.B[k], ax.is[k], ax.os[k]);
+        return lv;
+    };
+    if (kind != K_REALDFT && kind != K_INVREALDFT) {
+        for (int a = d - 1; a >= 0; --a) {
+            long long S;
+            Levels ax = axis_levels(Dm, a, S);
+            const bool first = (a == d - 1);
+            Levels lv = mk(ax, first ? ioff : ooff, ooff);
+            int rc = axis_pass(ws, pr, kind, ilog2i(Dm[a]), first ? in : (const void*)out, S, out, S, lv, st);
+            if (rc) return rc;
+        }
+        return 0;
+    }
+    const long long Nd = Dm[d - 1], H = Nd / 2 + 1, rows = P / Nd;
+    const int logNd = ilog2i(Nd);
+    std::vector<long long> Hm = Dm;
+    Hm[d - 1] = H;
+    if (kind == K_REALDFT) {
+        // real pass along the last axis (into tbuf when tbuf is given: the
+        // input and output overlap), then complex passes over the rest
+        void* dst1 = tbuf ? tbuf : out;
+        const long long* d1off = tbuf ? toff : ooff;
+        Levels ax;
+        if (rows > 1) ax.push((unsigned)rows, Nd, H);
+        int rc = axis_pass(ws, pr, K_REALDFT, logNd, in, 1, dst1, 1, mk(ax, ioff, d1off), st);
+        if (rc) return rc;
+        for (int a = d - 2; a >= 0; --a) {
+            long long S;
+            Levels a2 = axis_levels(Hm, a, S);
+            const bool first = (a == d - 2);
+            const void* src = first ? (const void*)dst1 : (const void*)out;
+            rc = axis_pass(ws, pr, K_DFT, ilog2i(Dm[a]), src, S, out, S, mk(a2, first ? d1off : ooff, ooff), st);
+            if (rc) return rc;
+        }
+        return 0;
+    }
+    // inverse real DFT: complex passes over the leading axes into tbuf (the
+    // input is preserved), then the real pass along the last axis
+    const void* src = in;
+    const long long* soff = ioff;
+    for (int a = d - 2; a >= 0; --a) {
+        long long S;
+        Levels a2 = axis_levels(Hm, a, S);
+        const bool first = (a == d - 2);
+        int rc = axis_pass(ws, pr, K_INVDFT, ilog2i(Dm[a]), first ? in : (const void*)tbuf, S, tbuf, S,
+                           mk(a2, first ? ioff : toff, toff), st);
+        if (rc) return rc;
+        src = tbuf;
+        soff = toff;
+    }
+    Levels ax;
+    if (rows > 1) ax.push((unsigned)rows, H, Nd);
+    return axis_pass(ws, pr, K_INVREALDFT, logNd, src, 1, out, 1, mk(ax, soff, ooff), st);
+}
+
+int enqueue(Plan& p, int kind, const void* in, void* out, hipStream_t st) {
+    if (family_of(kind) != p.fam) {
+        set_error("aux data does not match the transform kind");
+        return -1;
+    }
+    void* tbuf = nullptr;
+    if (kind == K_REALDFT && p.d > 1 && overlap(in, in_bytes(p, kind), out, out_bytes(p, kind)))
+        tbuf = p.tbuf;
+    if (kind == K_INVREALDFT && p.d > 1) tbuf = p.tbuf;
+    return run_entries(p.ws, p.prec, kind, p.d, p.Ns.data(), in, out, 1, nullptr, nullptr, nullptr, tbuf,
+                       nullptr, st);
+}
+
+int execute(Plan& p, int kind, const void* in, void* out) {
+    clear_error();
+    if (family_of(kind) != p.fam) {
+        set_error("aux data does not match the transform kind");
+        return -1;
+    }
+    int cur = -1;
+    if (hipGetDevice(&cur) != hipSuccess || cur != p.dev) (void)hipSetDevice(p.dev);
+    const size_t ib = in_bytes(p, kind), ob = out_bytes(p, kind);
+    if (is_device_pointer(in) && is_device_pointer(out)) {
+        int rc = enqueue(p, kind, in, out, p.stream);
+        HIPCHK(hipStreamSynchronize(p.stream), "transform");
+        return rc;
+    }
+    const size_t need = std::max(ib, ob);
+    if (p.stage_bytes < need) {
+        if (p.stage) (void)hipFree(p.stage);
+        p.stage = nullptr;
+        p.stage_bytes = 0;
+        HIPCHK(hipMalloc(&p.stage, need), "hipMalloc staging buffer");
+        p.stage_bytes = need;
+    }
+    HIPCHK(hipMemcpyAsync(p.stage, in, ib, hipMemcpyDefault, p.stream), "copy in");
+    int rc = enqueue(p, kind, p.stage, p.stage, p.stream);
+    if (rc) {
+        (void)hipStreamSynchronize(p.stream);
+        return rc;
+    }
+    HIPCHK(hipMemcpyAsync(out, p.stage, ob, hipMemcpyDefault, p.stream), "copy out");
+    HIPCHK(hipStreamSynchronize(p.stream), "transform");
+    return 0;
+}
+
+}  // namespace mfx

@@ -1,0 +1,123 @@
+// engine.hpp -- host side of the MI355X transform engine: plans ("aux"),
+// pass scheduling and kernel launches.  Precision-generic; the two public
+// C-ABI layers (FP64 and FP32 minfft.h) and the batch extension sit on top.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <string>
+#include <vector>
+
+#include "kinds.hpp"
+
+namespace mfx {
+
+enum Prec : int { PREC_F64 = 0, PREC_F32 = 1 };
+enum Family : int { FAM_DFT = 0, FAM_REALDFT = 1, FAM_T2T3 = 2, FAM_T4 = 3 };
+
+int family_of(int kind);
+
+// Batch levels of one axis pass, slowest first; level 0 may use per-entry
+// offset tables (device arrays) instead of a stride.
+struct Levels {
+    int n = 0;
+    unsigned B[4] = {1, 1, 1, 1};
+    long long is[4] = {0, 0, 0, 0};
+    long long os[4] = {0, 0, 0, 0};
+    const long long* ioff = nullptr;
+    const long long* ooff = nullptr;
+    const long long* foff = nullptr;   // two-step scratch offsets for the table level
+    void push(unsigned b, long long i, long long o) {
+        B[n] = b;
+        is[n] = i;
+        os[n] = o;
+        ++n;
+    }
+    unsigned long long count() const {
+        unsigned long long c = 1;
+        for (int k = 0; k < n; ++k) c *= B[k];
+        return c;
+    }
+};
+
+struct Device {
+    int id = -1;
+    int lds_max = 65536;
+    int ncu = 0;
+    int nxcd = 8;
+    int maxlog[2] = {0, 0};  // effective single-tile limit per precision
+};
+
+// Device resources shared by a plan's passes.
+struct Workspace {
+    void* tw = nullptr;        // table in the plan's precision
+    void* tw64 = nullptr;      // FP64 master table
+    int tw_log = 0;            // covers M <= 2^tw_log
+    void* fbuf = nullptr;      // two-step intermediate
+    void* gbuf = nullptr;      // two-step complex staging for real kinds
+    size_t fs_bytes = 0;
+};
+
+struct Plan {
+    unsigned magic = 0x6d667870u;
+    Family fam = FAM_DFT;
+    Prec prec = PREC_F64;
+    int d = 0;
+    std::vector<int> Ns;
+    long long P = 0;          // product of dims
+    int dev = -1;
+    Workspace ws;
+    void* tbuf = nullptr;     // full-array scratch ("t" of the reference aux)
+    size_t tbuf_bytes = 0;
+    void* stage = nullptr;    // staging buffer for host pointers (lazy)
+    size_t stage_bytes = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = true;
+};
+
+// error channel (thread-local)
+void set_error(const std::string& msg);
+const char* last_error();
+void clear_error();
+
+Device& device();
+bool ensure_device();
+
+size_t real_size(Prec p);
+Plan* make_plan(Family fam, Prec prec, int d, const int* Ns);
+void free_plan(Plan* p);
+
+// sizes in bytes of the input / output of `kind` for this plan
+size_t in_bytes(const Plan& p, int kind);
+size_t out_bytes(const Plan& p, int kind);
+
+// run `kind` on device pointers, asynchronously on `st`
+int enqueue(Plan& p, int kind, const void* in, void* out, hipStream_t st);
+// run `kind` with minfft semantics: any pointer kind, synchronous
+int execute(Plan& p, int kind, const void* in, void* out);
+
+// building blocks shared with the batch extension
+int alloc_twiddles(Workspace& ws, Prec prec, int log_m, hipStream_t st);
+void free_workspace(Workspace& ws);
+int needed_tw_log(int kind, int logN);
+// one 1D pass of `kind` (length N = 2^logN along the axis) over a batch
+int axis_pass(const Workspace& ws, Prec prec, int kind, int logN,
+              const void* in, long long in_e, void* out, long long out_e,
+              const Levels& lv, hipStream_t st);
+// scratch (bytes) an axis pass of this shape needs in ws.fbuf / ws.gbuf;
+// in_extent = number of input items addressed (for the mirrored layout)
+size_t axis_scratch_bytes(Prec prec, int kind, int logN, unsigned long long ntrans,
+                          unsigned long long in_extent);
+
+bool is_device_pointer(const void* p);
+
+// all passes of `kind` on `count` entries of dims Ns[0..d-1]; with tables
+// (ioff/ooff: per-entry item offsets into in/out, foff: into the two-step
+// scratch, toff: into tbuf) or, when they are null, a single transform at
+// in/out.  tbuf: complex scratch of the leading-axes passes of the inverse
+// real DFT (and of the aliased real DFT).
+int run_entries(const Workspace& ws, Prec pr, int kind, int d, const int* Ns, const void* in, void* out,
+                unsigned count, const long long* ioff, const long long* ooff, const long long* foff,
+                void* tbuf, const long long* toff, hipStream_t st);
+
+}  // namespace mfx

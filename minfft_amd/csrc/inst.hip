@@ -1,0 +1,53 @@
+// inst.hip -- instantiates the tile, pre, post and length-1 kernels of one
+// (precision, kind) pair.  Compiled once per pair with
+//   -DINST_F64=0|1 -DINST_KIND=<0..9>
+// so the 20 units build in parallel.
+#include "tile.hpp"
+#include "ktable.hpp"
+
+#ifndef INST_F64
+#error "INST_F64 must be defined"
+#endif
+#ifndef INST_KIND
+#error "INST_KIND must be defined"
+#endif
+
+namespace mfx {
+
+#if INST_F64
+using IT = double;
+constexpr int IMAX = kMaxLogF64;
+#define REGNAME(k) register_f64_k##k
+#else
+using IT = float;
+constexpr int IMAX = kMaxLogF32;
+#define REGNAME(k) register_f32_k##k
+#endif
+
+template <int LOGL>
+static void reg_tiles(KTable& t) {
+    if constexpr (LOGL <= IMAX) {
+        constexpr int E = default_E(LOGL);
+        t.tile[INST_KIND][LOGL] = reinterpret_cast<const void*>(&tile_kernel<IT, LOGL, E, INST_KIND>);
+        t.tile_E[LOGL] = E;
+        reg_tiles<LOGL + 1>(t);
+    }
+}
+
+#define CAT2(a, b) a##b
+#define CAT(a, b) CAT2(a, b)
+#if INST_F64
+#define FNAME CAT(register_f64_k, INST_KIND)
+#else
+#define FNAME CAT(register_f32_k, INST_KIND)
+#endif
+
+void FNAME(KTable& t) {
+    reg_tiles<0>(t);
+    t.maxlog = IMAX;
+    t.pre[INST_KIND] = reinterpret_cast<const void*>(&pre_kernel<IT, INST_KIND>);
+    t.post[INST_KIND] = reinterpret_cast<const void*>(&post_kernel<IT, INST_KIND>);
+    t.n1[INST_KIND] = reinterpret_cast<const void*>(&n1_kernel<IT, INST_KIND>);
+}
+
+}  // namespace mfx

@@ -1,0 +1,26 @@
+// ktable.hpp -- registry of the compiled kernel instances, filled by the
+// per-(precision, kind) instantiation units (inst.hip compiled once per pair).
+#pragma once
+#include "kinds.hpp"
+
+namespace mfx {
+
+constexpr int kMaxLog = 14;  // largest single-tile FFT length compiled: 2^14
+
+struct KTable {
+    const void* tile[K_COUNT][kMaxLog + 1];  // tile_kernel<T, logL, E, kind>
+    int tile_E[kMaxLog + 1];                  // points per thread per logL
+    int maxlog;                               // largest logL instantiated
+    const void* pre[K_COUNT];
+    const void* post[K_COUNT];
+    const void* n1[K_COUNT];
+};
+
+// maximum single-tile logL compiled per precision
+constexpr int kMaxLogF32 = 14;  // 16384 x 8 B = 128 KiB + padding
+constexpr int kMaxLogF64 = 13;  //  8192 x 16 B = 128 KiB + padding
+
+void register_f32(KTable& t);
+void register_f64(KTable& t);
+
+}  // namespace mfx

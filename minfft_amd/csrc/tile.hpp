@@ -1,0 +1,286 @@
+// tile.hpp -- the fused tile kernel: C independent 1D transforms of length
+// L = 2^LOGL per work-group, with the kind's pre-processing fused in front of
+// the FFT core and its post-processing fused behind it.
+//
+// Addressing.  A pass covers a batch of transforms described by up to four
+// batch levels (level 3 varies fastest).  Transform b of the batch has index
+// (i0,i1,i2,i3) and its items sit at
+//     base + e * estride,   base = sum_l i_l * stride_l   (or, for level 0,
+//     an entry-offset table when one is given: batched uneven workloads),
+// separately for the input and the output side, in units of that side's item
+// (complex point for complex sides, real value for real sides).
+//
+// Lane mapping.  Either "row" mapping (the TPT threads of one transform are
+// adjacent lanes: coalesced when estride == 1) or "c-fast" mapping (lane
+// neighbours work on neighbouring transforms: coalesced when the fastest
+// batch level has stride 1, i.e. strided columns of a multi-dimensional
+// array).  Each side is either read/written directly from registers with
+// the core's mapping, or staged through LDS with its own mapping.
+#pragma once
+#include "kinds.hpp"
+
+namespace mfx {
+
+enum IoMode : int { IO_DIRECT = 0, IO_STAGE_ROW = 1, IO_STAGE_CF = 2 };
+
+struct TileArgs {
+    const void* in;
+    void* out;
+    const void* tw;                    // twiddle table in the kernel's precision
+    const long long* in_off;           // optional level-0 entry offsets (input side)
+    const long long* out_off;          // optional level-0 entry offsets (output side)
+    long long in_s[4], out_s[4];       // batch strides
+    long long in_e, out_e;             // item strides
+    unsigned B[4];                     // batch extents
+    unsigned ntrans;                   // product of B
+    unsigned ntiles;                   // grid size
+    int C;                             // transforms per tile
+    int cfast;                         // core lane mapping: 1 = c-fast
+    int in_mode, out_mode;             // IoMode
+    int fs_level;                      // two-step twiddle: batch level holding n2
+    int fs_logN;                       // 0: none; else multiply by W_{2^fs_logN}^{n2*k}
+    int sc;                            // LDS slot length (complex points)
+    int xcd;                           // number of XCDs for the tile swizzle (1 = off)
+};
+
+struct TransformRef {
+    long long ib, ob;
+    unsigned i0, i1, i2, i3;
+    bool valid;
+    // batch index at level l (a select chain: no private-memory array)
+    __device__ __forceinline__ unsigned at(int l) const {
+        return l == 3 ? i3 : (l == 2 ? i2 : (l == 1 ? i1 : i0));
+    }
+};
+
+__device__ __forceinline__ TransformRef locate(const TileArgs& A, unsigned b) {
+    TransformRef r;
+    r.valid = b < A.ntrans;
+    if (!r.valid) b = 0;
+    unsigned i3 = b % A.B[3];
+    unsigned t = b / A.B[3];
+    unsigned i2 = t % A.B[2];
+    t /= A.B[2];
+    unsigned i1 = t % A.B[1];
+    unsigned i0 = t / A.B[1];
+    r.i0 = i0; r.i1 = i1; r.i2 = i2; r.i3 = i3;
+    long long ib = (A.in_off ? A.in_off[i0] : (long long)i0 * A.in_s[0]) +
+                   (long long)i1 * A.in_s[1] + (long long)i2 * A.in_s[2] + (long long)i3 * A.in_s[3];
+    long long ob = (A.out_off ? A.out_off[i0] : (long long)i0 * A.out_s[0]) +
+                   (long long)i1 * A.out_s[1] + (long long)i2 * A.out_s[2] + (long long)i3 * A.out_s[3];
+    r.ib = ib;
+    r.ob = ob;
+    return r;
+}
+
+// Remap the hardware work-group id so that consecutive tiles (neighbouring
+// columns in memory) land on the same XCD and share its L2: work-groups are
+// dispatched round-robin over the XCDs.
+__device__ __forceinline__ unsigned tile_id(const TileArgs& A) {
+    unsigned g = blockIdx.x;
+    const unsigned X = (unsigned)A.xcd;
+    if (X <= 1) return g;
+    const unsigned per = A.ntiles / X;
+    if (g >= per * X) return g;  // tail keeps identity
+    return (g % X) * per + g / X;
+}
+
+// Largest work-group of a tile kernel: FP64 radix-16 butterflies need more
+// than the 128 VGPRs a 1024-thread group leaves each lane.
+template <typename T> constexpr int max_threads() { return sizeof(T) == 8 ? 512 : 1024; }
+
+template <typename T, int LOGL, int E, int KIND>
+__global__ void __launch_bounds__(max_threads<T>()) tile_kernel(TileArgs A) {
+    using C2 = c2<T>;
+    constexpr int L = 1 << LOGL;
+    constexpr int TPT = L / E;
+    constexpr bool INV = kind_inverse_fft(KIND);
+    constexpr bool CIN = kind_complex_in(KIND);
+    constexpr bool COUT = kind_complex_out(KIND);
+    constexpr int NIN = kind_nin(KIND, L);
+    constexpr int NOUT = kind_nout(KIND, L);
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    C2* lds = reinterpret_cast<C2*>(smem_raw);
+    const C2* __restrict__ tw = reinterpret_cast<const C2*>(A.tw);
+
+    const int t = threadIdx.x;
+    const int C = A.C;
+    const unsigned tile = tile_id(A);
+    // the two lane mappings
+    const int c_row = t / TPT, tl_row = t - c_row * TPT;
+    const int c_cf = t % C, tl_cf = t / C;
+    const int c = A.cfast ? c_cf : c_row;
+    const int tl = A.cfast ? tl_cf : tl_row;
+    const TransformRef me = locate(A, tile * (unsigned)C + (unsigned)c);
+    C2* slot = lds + (long long)c * A.sc;
+
+    C2 v[E];
+
+    // ---------------- input side ----------------
+    if (A.in_mode == IO_DIRECT) {
+        // complex input read straight into the owned positions
+        const C2* __restrict__ in = reinterpret_cast<const C2*>(A.in);
+#pragma unroll
+        for (int m = 0; m < E; ++m) {
+            const int p = tl + m * TPT;
+            v[m] = me.valid ? in[me.ib + (long long)p * A.in_e] : mk<T>(0, 0);
+        }
+        if constexpr (KIND == K_REALDFT) {
+            // not reachable: real input is always staged
+        }
+    } else {
+        const bool rowm = (A.in_mode == IO_STAGE_ROW);
+        const int sc_ = rowm ? c_row : c_cf;
+        const int stl = rowm ? tl_row : tl_cf;
+        const TransformRef src = (sc_ == c) ? me : locate(A, tile * (unsigned)C + (unsigned)sc_);
+        C2* sslot = lds + (long long)sc_ * A.sc;
+        if (src.valid) {
+            if constexpr (CIN || KIND == K_REALDFT) {
+                // complex items (for the real DFT: pairs x[2p], x[2p+1])
+                const C2* __restrict__ in = reinterpret_cast<const C2*>(A.in);
+                const long long es = (KIND == K_REALDFT) ? 1 : A.in_e;
+                const long long base = (KIND == K_REALDFT) ? (src.ib >> 1) : src.ib;
+                for (int i = stl; i < NIN; i += TPT) sslot[lpad(i)] = in[base + (long long)i * es];
+            } else {
+                const T* __restrict__ in = reinterpret_cast<const T*>(A.in);
+                T* rs = reinterpret_cast<T*>(sslot);
+                for (int i = stl; i < NIN; i += TPT)
+                    rs[2 * lpad(i >> 1) + (i & 1)] = in[src.ib + (long long)i * A.in_e];
+            }
+        }
+        __syncthreads();
+        const T* rslot = reinterpret_cast<const T*>(slot);
+        auto X = [&](int i) -> T { return rslot[2 * lpad(i >> 1) + (i & 1)]; };
+        auto Z = [&](int i) -> C2 { return slot[lpad(i)]; };
+#pragma unroll
+        for (int m = 0; m < E; ++m) {
+            const int p = tl + m * TPT;
+            v[m] = pre_point<T, KIND>(p, L, tw, X, Z);
+        }
+        __syncthreads();
+    }
+
+    // ---------------- FFT core ----------------
+    Core<T, LOGL, E, INV>::run(v, slot, tl, tw);
+
+    // two-step twiddle: W_N^{n2*k} on output point k
+    if (A.fs_logN > 0) {
+        const unsigned n2 = me.at(A.fs_level);
+        const unsigned Nfs = 1u << A.fs_logN;
+        const C2* twN = tw + Nfs;
+#pragma unroll
+        for (int m = 0; m < E; ++m) {
+            const unsigned p = (unsigned)(tl + m * TPT);
+            const C2 w = twN[(n2 * p) & (Nfs - 1)];
+            v[m] = INV ? cmulc<T>(v[m], w) : cmul<T>(v[m], w);
+        }
+    }
+
+    // ---------------- output side ----------------
+    if constexpr (KIND == K_DFT || KIND == K_INVDFT) {
+        if (A.out_mode == IO_DIRECT) {
+            C2* __restrict__ out = reinterpret_cast<C2*>(A.out);
+            if (me.valid) {
+#pragma unroll
+                for (int m = 0; m < E; ++m) {
+                    const int p = tl + m * TPT;
+                    out[me.ob + (long long)p * A.out_e] = v[m];
+                }
+            }
+            return;
+        }
+    }
+#pragma unroll
+    for (int m = 0; m < E; ++m) slot[lpad(tl + m * TPT)] = v[m];
+    __syncthreads();
+    {
+        const bool rowm = (A.out_mode != IO_STAGE_CF);
+        const int sc_ = rowm ? c_row : c_cf;
+        const int stl = rowm ? tl_row : tl_cf;
+        const TransformRef dst = (sc_ == c) ? me : locate(A, tile * (unsigned)C + (unsigned)sc_);
+        const C2* sslot = lds + (long long)sc_ * A.sc;
+        if (!dst.valid) return;
+        auto Tf = [&](int p) -> C2 { return sslot[lpad(p)]; };
+        if constexpr (COUT) {
+            C2* __restrict__ out = reinterpret_cast<C2*>(A.out);
+            for (int i = stl; i < NOUT; i += TPT)
+                out[dst.ob + (long long)i * A.out_e] = post_item<T, KIND>(i, L, tw, Tf);
+        } else {
+            T* __restrict__ out = reinterpret_cast<T*>(A.out);
+            for (int i = stl; i < NOUT; i += TPT)
+                out[dst.ob + (long long)i * A.out_e] = post_item<T, KIND>(i, L, tw, Tf).x;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Element-wise kernels for the two-step path of the real kinds and for
+// length-1 axes.  One thread per item; same addressing as the tile kernel
+// with C = 1 (so b = transform index).
+// ---------------------------------------------------------------------------
+
+// pre: kind's input -> complex FFT input (L points per transform) at
+// out (contiguous, transform b at b*L)
+template <typename T, int KIND>
+__global__ void __launch_bounds__(256) pre_kernel(TileArgs A, int L, long long total) {
+    using C2 = c2<T>;
+    const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= total) return;
+    const unsigned b = (unsigned)(g / L);
+    const int p = (int)(g - (long long)b * L);
+    const TransformRef r = locate(A, b);
+    const C2* __restrict__ tw = reinterpret_cast<const C2*>(A.tw);
+    const T* __restrict__ xr = reinterpret_cast<const T*>(A.in);
+    const C2* __restrict__ xc = reinterpret_cast<const C2*>(A.in);
+    auto X = [&](int i) -> T { return xr[r.ib + (long long)i * A.in_e]; };
+    auto Z = [&](int i) -> C2 {
+        if constexpr (KIND == K_REALDFT) {
+            const long long q = r.ib + 2LL * i * A.in_e;
+            return mk<T>(xr[q], xr[q + A.in_e]);
+        } else {
+            return xc[r.ib + (long long)i * A.in_e];
+        }
+    };
+    C2* __restrict__ out = reinterpret_cast<C2*>(A.out);
+    out[(long long)b * L + p] = pre_point<T, KIND>(p, L, tw, X, Z);
+}
+
+// post: complex FFT output (contiguous, transform b at b*L) -> kind's output
+template <typename T, int KIND>
+__global__ void __launch_bounds__(256) post_kernel(TileArgs A, int L, long long total) {
+    using C2 = c2<T>;
+    constexpr int NO = 1;  // placeholder to keep the signature uniform
+    (void)NO;
+    const int nout = kind_nout(KIND, L);
+    const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= total) return;
+    const unsigned b = (unsigned)(g / nout);
+    const int i = (int)(g - (long long)b * nout);
+    const TransformRef r = locate(A, b);
+    const C2* __restrict__ tw = reinterpret_cast<const C2*>(A.tw);
+    const C2* __restrict__ src = reinterpret_cast<const C2*>(A.in) + (long long)b * L;
+    auto Tf = [&](int p) -> C2 { return src[p]; };
+    C2 y = post_item<T, KIND>(i, L, tw, Tf);
+    if constexpr (kind_complex_out(KIND)) {
+        reinterpret_cast<C2*>(A.out)[r.ob + (long long)i * A.out_e] = y;
+    } else {
+        reinterpret_cast<T*>(A.out)[r.ob + (long long)i * A.out_e] = y.x;
+    }
+}
+
+// length-1 axis: one item per transform
+template <typename T, int KIND>
+__global__ void __launch_bounds__(256) n1_kernel(TileArgs A) {
+    using C2 = c2<T>;
+    const unsigned b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= A.ntrans) return;
+    const TransformRef r = locate(A, b);
+    C2 x;
+    if constexpr (kind_complex_in(KIND)) x = reinterpret_cast<const C2*>(A.in)[r.ib];
+    else x = mk<T>(reinterpret_cast<const T*>(A.in)[r.ib], (T)0);
+    C2 y = n1_item<T, KIND>(x);
+    if constexpr (kind_complex_out(KIND)) reinterpret_cast<C2*>(A.out)[r.ob] = y;
+    else reinterpret_cast<T*>(A.out)[r.ob] = y.x;
+}
+
+}  // namespace mfx

@@ -1,0 +1,61 @@
+"""The C-ABI boundary, checked without a GPU: the libraries are built, load,
+and export every symbol include/minfft.h and include/minfft_ext.h declare,
+with the reference's 27 names (aimukhin/minfft minfft.h:35-63)."""
+import ctypes
+import os
+import re
+import subprocess
+
+import pytest
+
+import minfft_amd as mf
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+INC = os.path.join(ROOT, "include")
+
+
+def _declared(header):
+    txt = open(os.path.join(INC, header)).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return set(re.findall(r"\b(minfft_\w+)\s*\(", txt))
+
+
+def _exported(lib):
+    out = subprocess.run(["nm", "-D", "--defined-only", mf.lib_path(lib)],
+                         capture_output=True, text=True, check=True).stdout
+    return {line.split()[-1] for line in out.splitlines() if " T " in line}
+
+
+def test_header_declares_the_reference_api():
+    names = _declared("minfft.h")
+    assert names == set(mf.SYMBOLS)
+    assert len(names) == 27
+
+
+@pytest.mark.parametrize("lib", ["libminfft.so", "libminfft_single.so"])
+def test_api_libraries_export_all_27_symbols(lib):
+    exp = {s for s in _exported(lib) if s.startswith("minfft_")}
+    assert exp == set(mf.SYMBOLS)
+
+
+def test_engine_exports_extension_symbols():
+    exp = _exported("libminfft_engine.so")
+    assert _declared("minfft_ext.h") <= exp
+    assert set(mf.EXT_SYMBOLS) <= exp
+    # the engine itself must not export the precision-specific API
+    assert not ({s for s in exp if s.startswith("minfft_")} & set(mf.SYMBOLS))
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+def test_libraries_load_through_ctypes(prec):
+    lib = mf.Minfft(prec)
+    for s in mf.SYMBOLS:
+        assert getattr(lib.lib, s)
+    assert mf.engine().minfft_last_error() is not None
+
+
+def test_kernels_built_for_gfx950():
+    """The engine's embedded code object targets gfx950 (this build's own
+    output, read as bytes)."""
+    data = open(mf.lib_path("libminfft_engine.so"), "rb").read()
+    assert b"amdgcn-amd-amdhsa--gfx950" in data

@@ -1,0 +1,216 @@
+"""GPU parity of the 27-symbol API (libminfft.so / libminfft_single.so)
+against the CPU oracle and the reference's golden vectors.
+
+Bars: FP64 rel-L2 <= 1e-14*log2(N) vs the FP64 oracle (which agrees with the
+reference to the last bit); FP32 rel-L2 <= 1e-6 vs the FP64 oracle on the same
+(float-rounded) input."""
+import os
+
+import numpy as np
+import pytest
+
+import minfft_amd as mf
+from oracle.oracle import KINDS, in_shape, out_shape, is_complex_in, is_complex_out
+from tests.helpers import TOL32, make_input, rel_l2, tol64
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+@pytest.fixture(scope="module")
+def libs():
+    return {64: mf.Minfft(64), 32: mf.Minfft(32)}
+
+
+def gpu_run(lib, kind, x, shape, inplace=False):
+    """Run through the C API on host (numpy) buffers."""
+    aux = mf.Aux(lib, mf.FAMILY[kind], shape)
+    rdt = np.float64 if lib.prec == 64 else np.float32
+    cdt = np.complex128 if lib.prec == 64 else np.complex64
+    odt = cdt if is_complex_out(kind) else rdt
+    try:
+        if not inplace:
+            y = np.empty(out_shape(kind, shape), dtype=odt)
+            aux(kind, np.ascontiguousarray(x), y)
+            return y
+        nbytes = max(x.nbytes, int(np.prod(out_shape(kind, shape))) * np.dtype(odt).itemsize)
+        buf = np.zeros(nbytes, dtype=np.uint8)
+        buf[: x.nbytes] = np.frombuffer(np.ascontiguousarray(x).tobytes(), dtype=np.uint8)
+        aux(kind, buf, buf)
+        n = int(np.prod(out_shape(kind, shape)))
+        return buf.view(odt)[:n].reshape(out_shape(kind, shape)).copy()
+    finally:
+        aux.close()
+
+
+def check(kind, shape, prec, y, x, oracle64):
+    """y (GPU) vs the FP64 oracle on the same input."""
+    yo = oracle64.run(kind, x.astype(np.complex128 if is_complex_in(kind) else np.float64), shape)
+    e = rel_l2(y, yo)
+    bar = tol64(shape) if prec == 64 else TOL32
+    assert e <= bar, f"{kind} {shape} prec={prec}: rel-L2 {e:.3e} > {bar:.1e}"
+    return e
+
+
+SHAPES = [(1,), (2,), (4,), (8,), (16,), (32,), (64,), (128,), (256,), (512,), (1024,), (2048,),
+          (4096,), (8192,), (16384,),
+          (2, 2), (4, 8), (8, 1), (1, 8), (64, 32), (32, 64), (256, 256), (2, 4, 8), (8, 8, 8),
+          (16, 1, 32), (32, 16, 8)]
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+@pytest.mark.parametrize("shape", SHAPES, ids=lambda s: "x".join(map(str, s)))
+def test_all_kinds_vs_oracle(libs, oracle64, prec, shape):
+    lib = libs[prec]
+    print(f"[case] all kinds prec={prec} shape={shape}", flush=True)
+    for i, kind in enumerate(KINDS):
+        x = make_input(kind, shape, prec, 100 + i, oracle64 if prec == 64 else None) \
+            if kind != "invrealdft" else None
+        if kind == "invrealdft":
+            xr = make_input("dct2", shape, prec, 100 + i)
+            z = oracle64.run("realdft", xr.astype(np.float64), shape)
+            x = z.astype(np.complex128 if prec == 64 else np.complex64)
+        y = gpu_run(lib, kind, x, shape)
+        check(kind, shape, prec, y, x, oracle64)
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+def test_golden_vectors(libs, oracle64, prec):
+    """Every golden vector of the reference (tests/golden) through the GPU."""
+    g = np.load(os.path.join(GOLDEN, f"golden_{prec}.npz"))
+    keys = sorted({k.rsplit("__", 1)[0] for k in g.files})
+    print(f"[case] golden prec={prec} ({len(keys)} vectors)", flush=True)
+    for key in keys:
+        kind, sk = key.split("__")
+        shape = tuple(int(s) for s in sk.split("x"))
+        y = gpu_run(libs[prec], kind, g[key + "__in"], shape)
+        e = rel_l2(y, g[key + "__out"])
+        bar = tol64(shape) if prec == 64 else 4 * TOL32
+        assert e <= bar, (key, e)
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+@pytest.mark.parametrize("shape", [(1,), (2,), (16,), (1024,), (8, 16), (4, 2, 8), (64, 64)],
+                         ids=lambda s: "x".join(map(str, s)))
+def test_in_place(libs, oracle64, prec, shape):
+    print(f"[case] in-place prec={prec} shape={shape}", flush=True)
+    for i, kind in enumerate(KINDS):
+        if kind == "invrealdft":
+            xr = make_input("dct2", shape, prec, 200 + i)
+            x = oracle64.run("realdft", xr.astype(np.float64), shape).astype(
+                np.complex128 if prec == 64 else np.complex64)
+        else:
+            x = make_input(kind, shape, prec, 200 + i)
+        y = gpu_run(libs[prec], kind, x, shape, inplace=True)
+        check(kind, shape, prec, y, x, oracle64)
+
+
+@pytest.mark.parametrize("prec,N", [(64, 16384), (64, 32768), (64, 131072), (32, 32768), (32, 65536),
+                                    (32, 262144)])
+def test_long_1d_two_step(libs, oracle64, prec, N):
+    """Lengths beyond one work-group's LDS take the two-step path."""
+    print(f"[case] two-step prec={prec} N={N}", flush=True)
+    for i, kind in enumerate(KINDS):
+        if kind == "invrealdft":
+            xr = make_input("dct2", (N,), prec, 300 + i)
+            x = oracle64.run("realdft", xr.astype(np.float64), (N,)).astype(
+                np.complex128 if prec == 64 else np.complex64)
+        else:
+            x = make_input(kind, (N,), prec, 300 + i)
+        y = gpu_run(libs[prec], kind, x, (N,))
+        check(kind, (N,), prec, y, x, oracle64)
+
+
+def test_device_pointers_zero_copy(libs, oracle64):
+    import torch
+    print("[case] device pointers", flush=True)
+    dev = torch.device("cuda:0")
+    for prec in (64, 32):
+        cdt = torch.complex128 if prec == 64 else torch.complex64
+        rdt = torch.float64 if prec == 64 else torch.float32
+        shape = (32, 64)
+        x = make_input("dft", shape, prec, 7)
+        xt = torch.from_numpy(x).to(dev)
+        yt = torch.empty(shape, dtype=cdt, device=dev)
+        aux = mf.Aux(libs[prec], "dft", shape)
+        aux("dft", xt, yt)
+        check("dft", shape, prec, yt.cpu().numpy(), x, oracle64)
+        aux.close()
+        xr = make_input("dct2", shape, prec, 8)
+        xrt = torch.from_numpy(xr).to(dev)
+        zt = torch.empty((32, 33), dtype=cdt, device=dev)
+        aux = mf.Aux(libs[prec], "realdft", shape)
+        aux("realdft", xrt, zt)
+        check("realdft", shape, prec, zt.cpu().numpy(), xr, oracle64)
+        back = torch.empty(shape, dtype=rdt, device=dev)
+        aux("invrealdft", zt, back)
+        n = 32 * 64
+        assert rel_l2(back.cpu().numpy() / n, xr) <= (1e-14 if prec == 64 else 1e-6)
+        aux.close()
+
+
+def test_round_trips_and_input_preserved(libs):
+    print("[case] round trips", flush=True)
+    lib = libs[64]
+    shape = (16, 32)
+    n = 16 * 32
+    x = make_input("dft", shape, 64, 9)
+    x0 = x.copy()
+    aux = mf.Aux(lib, "dft", shape)
+    y = np.empty_like(x)
+    aux("dft", x, y)
+    assert np.array_equal(x, x0)
+    back = np.empty_like(x)
+    aux("invdft", y, back)
+    assert rel_l2(back / n, x) <= 1e-14
+    aux.close()
+    xr = make_input("dct2", shape, 64, 10)
+    for a, b, fac in (("dct2", "dct3", 4 * n), ("dst2", "dst3", 4 * n), ("dct4", "dct4", 4 * n),
+                      ("dst4", "dst4", 4 * n)):
+        aux = mf.Aux(lib, mf.FAMILY[a], shape)
+        y1 = np.empty_like(xr)
+        aux(a, xr, y1)
+        y2 = np.empty_like(xr)
+        aux(b, y1, y2)
+        assert rel_l2(y2 / fac, xr) <= 1e-14, (a, b)
+        aux.close()
+    # multi-dimensional inverse real DFT preserves its input here
+    z = np.fft.rfftn(xr)
+    z0 = z.copy()
+    aux = mf.Aux(lib, "realdft", shape)
+    out = np.empty(shape)
+    aux("invrealdft", z, out)
+    assert np.array_equal(z, z0)
+    assert rel_l2(out / n, xr) <= 1e-14
+    aux.close()
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+def test_makers_reject_invalid_lengths(libs, prec):
+    print(f"[case] maker errors prec={prec}", flush=True)
+    lib = libs[prec]
+    for fam in ("dft", "realdft", "t2t3", "t4"):
+        for N in (0, -4, 3, 12, 1000):
+            assert not lib.mkaux_raw(f"minfft_mkaux_{fam}_1d", N)
+            assert "power" in mf.last_error() or "invalid" in mf.last_error()
+        assert not lib.mkaux_raw(f"minfft_mkaux_{fam}_2d", 8, 6)
+        assert not lib.mkaux_raw(f"minfft_mkaux_{fam}_3d", 8, 8, 0)
+        import ctypes
+        Ns = (ctypes.c_int * 1)(8)
+        assert not lib.mkaux_raw(f"minfft_mkaux_{fam}", 0, Ns)
+    lib.free_aux(None)
+
+
+def test_mismatched_aux_is_reported(libs):
+    lib = libs[64]
+    aux = mf.Aux(lib, "dft", (16,))
+    x = np.zeros(16)
+    with pytest.raises(mf.MinfftError):
+        lib.run_ptr("dct2", x.ctypes.data, x.ctypes.data, aux.h)
+    aux.close()
+    # an aux of the FP64 library is refused by the FP32 library
+    aux = mf.Aux(lib, "dft", (16,))
+    with pytest.raises(mf.MinfftError):
+        libs[32].run_ptr("dft", x.ctypes.data, x.ctypes.data, aux.h)
+    aux.close()
