@@ -55,8 +55,12 @@ def gpu_batch(sizes, prec, steps, warmup, torch, mf, seed=SEED):
     xout = torch.empty(total, dtype=cdt, device="cuda")
     entries = [("dft", (int(n),), int(o) * esz, int(o) * esz) for n, o in zip(sizes, offs)]
     plan = mf.Batch(entries, prec)
-    st = torch.cuda.current_stream()
+    # a stream of our own (default priority): its handle is what the graph
+    # launches on and what the events below are recorded on
+    st = torch.cuda.Stream()
     sh = st.cuda_stream
+    assert sh, "need a non-null stream handle"
+    torch.cuda.synchronize()
     for _ in range(max(warmup, 1)):
         plan.run(xin.data_ptr(), xout.data_ptr(), sh)
     torch.cuda.synchronize()

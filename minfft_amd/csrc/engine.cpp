@@ -13,8 +13,10 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
+#include <string>
 
 #include "ktable.hpp"
 #include "tile.hpp"
@@ -133,6 +135,45 @@ bool is_device_pointer(const void* p) {
            a.type == hipMemoryTypeUnified;
 }
 
+// ------------------------------------------------------------ tile tuning
+// Launch-shape parameters of the tile kernel.  Defaults are the measured
+// best on MI355X; MINFFT_TUNE="key=value,..." overrides them for sweeps
+// (keys: tpw = target threads per work-group for row tiles, cfc = columns per
+// c-fast tile in bytes/8, xcd = 0|1 XCD-aware tile order, stage = 0|1 force
+// LDS-staged complex I/O).
+struct Tune {
+    int tpw = 256;
+    int cfb = 128;  // bytes per row segment of a c-fast tile
+    int xcd = 1;
+    int stage = 0;
+};
+static Tune g_tune;
+static std::once_flag g_tune_once;
+static const Tune& tune() {
+    std::call_once(g_tune_once, [] {
+        const char* e = std::getenv("MINFFT_TUNE");
+        if (!e) return;
+        std::string s(e);
+        size_t pos = 0;
+        while (pos < s.size()) {
+            size_t c = s.find(',', pos);
+            if (c == std::string::npos) c = s.size();
+            std::string kv = s.substr(pos, c - pos);
+            size_t eq = kv.find('=');
+            if (eq != std::string::npos) {
+                std::string k = kv.substr(0, eq);
+                int v = std::atoi(kv.c_str() + eq + 1);
+                if (k == "tpw" && v > 0) g_tune.tpw = v;
+                else if (k == "cfb" && v > 0) g_tune.cfb = v;
+                else if (k == "xcd") g_tune.xcd = v;
+                else if (k == "stage") g_tune.stage = v;
+            }
+            pos = c + 1;
+        }
+    });
+    return g_tune;
+}
+
 // -------------------------------------------------------------- launching
 static std::mutex g_attr_mu;
 
@@ -209,7 +250,12 @@ static int launch_tile(Prec prec, int kind, int logL, const void* tw, const void
     const bool in_cf = f >= 0 && !tab_f && A.in_s[f] == 1;
     const bool out_cf = f >= 0 && !tab_f && A.out_s[f] == 1;
     const bool cplx = (kind == K_DFT || kind == K_INVDFT);
-    if (cplx) {
+    const Tune& tn = tune();
+    if (cplx && tn.stage) {
+        A.in_mode = in_row ? IO_STAGE_ROW : IO_STAGE_CF;
+        A.out_mode = out_row ? IO_STAGE_ROW : IO_STAGE_CF;
+        A.cfast = (A.in_mode == IO_STAGE_CF) ? 1 : 0;
+    } else if (cplx) {
         if (in_row && out_row) { A.cfast = 0; A.in_mode = IO_DIRECT; A.out_mode = IO_DIRECT; }
         else if (in_cf && out_cf) { A.cfast = 1; A.in_mode = IO_DIRECT; A.out_mode = IO_DIRECT; }
         else if (out_cf) { A.cfast = 1; A.out_mode = IO_DIRECT; A.in_mode = in_row ? IO_STAGE_ROW : IO_DIRECT; }
@@ -227,10 +273,10 @@ static int launch_tile(Prec prec, int kind, int logL, const void* tw, const void
     int C;
     if (wants_cf) {
         // several columns per tile so each row segment spans >= 64 B
-        C = (int)(128 / cs);
+        C = std::max(1, (int)(tn.cfb / cs));
         while (C > 1 && C * TPT > maxT) C >>= 1;
     } else {
-        C = std::max(1, 256 / TPT);
+        C = std::max(1, std::min(tn.tpw, maxT) / TPT);
     }
     if ((unsigned long long)C > ntr) C = (int)ntr;
     size_t budget = (size_t)D.lds_max;
@@ -243,7 +289,7 @@ static int launch_tile(Prec prec, int kind, int logL, const void* tw, const void
     A.sc = sc;
     const unsigned long long tiles = (ntr + C - 1) / C;
     A.ntiles = (unsigned)tiles;
-    A.xcd = (tiles >= 2ull * D.nxcd) ? D.nxcd : 1;
+    A.xcd = (tn.xcd && wants_cf && tiles >= 2ull * D.nxcd) ? D.nxcd : 1;
     void* args[] = {&A};
     return launch_fn(fn, (unsigned)tiles, (unsigned)(C * TPT), args, (size_t)C * sc * cs, st);
 }

@@ -153,19 +153,67 @@ struct Core {
     static constexpr int NP = (LOGL == 0) ? 0 : (LOGL + LOGE - 1) / LOGE;
     static constexpr int REM = (LOGL == 0) ? 0 : LOGL - (NP - 1) * LOGE;
 
+    // Twiddles of pass P: W_{Ns*R}^{r*k} for r = 1..R-1 of each of the NB
+    // butterflies.  Loaded one pass ahead (before the LDS exchange that
+    // precedes the pass) so their latency hides behind the exchange.
+    template <int P> struct Tw {
+        static constexpr int LOGR = (P == 0) ? REM : LOGE;
+        static constexpr int LOGNS = (P == 0) ? 0 : REM + (P - 1) * LOGE;
+        static constexpr int R = 1 << LOGR;
+        static constexpr int Ns = 1 << LOGNS;
+        static constexpr int NB = E / R;
+        static constexpr int CNT = (Ns > 1) ? NB * (R - 1) : 1;
+        c2<T> w[CNT];
+        __device__ __forceinline__ void load(int tl, const c2<T>* __restrict__ tw) {
+            if constexpr (Ns > 1) {
+                const c2<T>* twr = tw + Ns * R;
+#pragma unroll
+                for (int b = 0; b < NB; ++b) {
+                    const int k = (tl + b * TPT) & (Ns - 1);
+#pragma unroll
+                    for (int r = 1; r < R; ++r) w[b * (R - 1) + r - 1] = twr[r * k];
+                }
+            }
+        }
+    };
+
+    // owned positions tl + m*TPT of the slot <-> registers
+    __device__ static __forceinline__ void read_owned(c2<T> (&v)[E], const c2<T>* slot, int tl) {
+        if constexpr (TPT % 16 == 0) {
+            const c2<T>* sb = slot + lpad(tl);
+#pragma unroll
+            for (int m = 0; m < E; ++m) v[m] = sb[m * (TPT + TPT / 16)];
+        } else {
+#pragma unroll
+            for (int m = 0; m < E; ++m) v[m] = slot[lpad(tl + m * TPT)];
+        }
+    }
+    __device__ static __forceinline__ void write_owned(const c2<T> (&v)[E], c2<T>* slot, int tl) {
+        if constexpr (TPT % 16 == 0) {
+            c2<T>* sb = slot + lpad(tl);
+#pragma unroll
+            for (int m = 0; m < E; ++m) sb[m * (TPT + TPT / 16)] = v[m];
+        } else {
+#pragma unroll
+            for (int m = 0; m < E; ++m) slot[lpad(tl + m * TPT)] = v[m];
+        }
+    }
+
     // slot: this transform's LDS slot (complex points, padded by lpad).
     // On entry the slot may be overwritten (caller has synchronised).
     __device__ static void run(c2<T> (&v)[E], c2<T>* slot, int tl, const c2<T>* __restrict__ tw) {
-        if constexpr (NP > 0) pass<0>(v, slot, tl, tw);
+        if constexpr (NP > 0) {
+            Tw<0> w0;
+            pass<0>(v, slot, tl, tw, w0);
+        }
     }
 
     template <int P>
-    __device__ static void pass(c2<T> (&v)[E], c2<T>* slot, int tl, const c2<T>* __restrict__ tw) {
-        constexpr int LOGR = (P == 0) ? REM : LOGE;
-        constexpr int LOGNS = (P == 0) ? 0 : REM + (P - 1) * LOGE;
-        constexpr int R = 1 << LOGR;
-        constexpr int Ns = 1 << LOGNS;
-        constexpr int NB = E / R;
+    __device__ static void pass(c2<T> (&v)[E], c2<T>* slot, int tl, const c2<T>* __restrict__ tw,
+                                const Tw<P>& twp) {
+        constexpr int R = Tw<P>::R;
+        constexpr int Ns = Tw<P>::Ns;
+        constexpr int NB = Tw<P>::NB;
         constexpr bool last = (P == NP - 1);
 #pragma unroll
         for (int b = 0; b < NB; ++b) {
@@ -175,10 +223,9 @@ struct Core {
 #pragma unroll
             for (int r = 0; r < R; ++r) a[r] = v[b + r * NB];
             if constexpr (Ns > 1) {
-                const c2<T>* twr = tw + Ns * R;
 #pragma unroll
                 for (int r = 1; r < R; ++r) {
-                    c2<T> w = twr[r * k];
+                    const c2<T> w = twp.w[b * (R - 1) + r - 1];
                     a[r] = INV ? cmulc<T>(a[r], w) : cmul<T>(a[r], w);
                 }
             }
@@ -188,16 +235,25 @@ struct Core {
                 for (int r = 0; r < R; ++r) v[b + r * NB] = a[r];
             } else {
                 const int base = (j - k) * R + k;
+                if constexpr (Ns == 1 || (Ns % 16) == 0) {
+                    // padded index is linear in r: one base, immediate offsets
+                    c2<T>* sb = slot + lpad(base);
+                    constexpr int step = (Ns == 1) ? 1 : Ns + Ns / 16;
 #pragma unroll
-                for (int r = 0; r < R; ++r) slot[lpad(base + r * Ns)] = a[r];
+                    for (int r = 0; r < R; ++r) sb[r * step] = a[r];
+                } else {
+#pragma unroll
+                    for (int r = 0; r < R; ++r) slot[lpad(base + r * Ns)] = a[r];
+                }
             }
         }
         if constexpr (!last) {
+            Tw<P + 1> nxt;
+            nxt.load(tl, tw);
             __syncthreads();
-#pragma unroll
-            for (int m = 0; m < E; ++m) v[m] = slot[lpad(tl + m * TPT)];
+            read_owned(v, slot, tl);
             __syncthreads();
-            pass<P + 1>(v, slot, tl, tw);
+            pass<P + 1>(v, slot, tl, tw, nxt);
         }
     }
 };

@@ -119,14 +119,21 @@ __global__ void __launch_bounds__(max_threads<T>()) tile_kernel(TileArgs A) {
     // ---------------- input side ----------------
     if (A.in_mode == IO_DIRECT) {
         // complex input read straight into the owned positions
-        const C2* __restrict__ in = reinterpret_cast<const C2*>(A.in);
+        const C2* __restrict__ in = reinterpret_cast<const C2*>(A.in) + me.ib;
+        if (me.valid) {
+            if (A.in_e == 1) {
 #pragma unroll
-        for (int m = 0; m < E; ++m) {
-            const int p = tl + m * TPT;
-            v[m] = me.valid ? in[me.ib + (long long)p * A.in_e] : mk<T>(0, 0);
-        }
-        if constexpr (KIND == K_REALDFT) {
-            // not reachable: real input is always staged
+                for (int m = 0; m < E; ++m) v[m] = in[tl + m * TPT];
+            } else {
+                const long long es = A.in_e;
+                const C2* __restrict__ b = in + (long long)tl * es;
+                const long long step = es * TPT;
+#pragma unroll
+                for (int m = 0; m < E; ++m) v[m] = b[m * step];
+            }
+        } else {
+#pragma unroll
+            for (int m = 0; m < E; ++m) v[m] = mk<T>(0, 0);
         }
     } else {
         const bool rowm = (A.in_mode == IO_STAGE_ROW);
@@ -152,10 +159,14 @@ __global__ void __launch_bounds__(max_threads<T>()) tile_kernel(TileArgs A) {
         const T* rslot = reinterpret_cast<const T*>(slot);
         auto X = [&](int i) -> T { return rslot[2 * lpad(i >> 1) + (i & 1)]; };
         auto Z = [&](int i) -> C2 { return slot[lpad(i)]; };
+        if constexpr (KIND == K_DFT || KIND == K_INVDFT || KIND == K_REALDFT) {
+            Core<T, LOGL, E, INV>::read_owned(v, slot, tl);
+        } else {
 #pragma unroll
-        for (int m = 0; m < E; ++m) {
-            const int p = tl + m * TPT;
-            v[m] = pre_point<T, KIND>(p, L, tw, X, Z);
+            for (int m = 0; m < E; ++m) {
+                const int p = tl + m * TPT;
+                v[m] = pre_point<T, KIND>(p, L, tw, X, Z);
+            }
         }
         __syncthreads();
     }
@@ -179,19 +190,23 @@ __global__ void __launch_bounds__(max_threads<T>()) tile_kernel(TileArgs A) {
     // ---------------- output side ----------------
     if constexpr (KIND == K_DFT || KIND == K_INVDFT) {
         if (A.out_mode == IO_DIRECT) {
-            C2* __restrict__ out = reinterpret_cast<C2*>(A.out);
+            C2* __restrict__ out = reinterpret_cast<C2*>(A.out) + me.ob;
             if (me.valid) {
+                if (A.out_e == 1) {
 #pragma unroll
-                for (int m = 0; m < E; ++m) {
-                    const int p = tl + m * TPT;
-                    out[me.ob + (long long)p * A.out_e] = v[m];
+                    for (int m = 0; m < E; ++m) out[tl + m * TPT] = v[m];
+                } else {
+                    const long long es = A.out_e;
+                    C2* __restrict__ b = out + (long long)tl * es;
+                    const long long step = es * TPT;
+#pragma unroll
+                    for (int m = 0; m < E; ++m) b[m * step] = v[m];
                 }
             }
             return;
         }
     }
-#pragma unroll
-    for (int m = 0; m < E; ++m) slot[lpad(tl + m * TPT)] = v[m];
+    Core<T, LOGL, E, INV>::write_owned(v, slot, tl);
     __syncthreads();
     {
         const bool rowm = (A.out_mode != IO_STAGE_CF);
