@@ -31,7 +31,7 @@ COMMON = [
     "-Wall", "-Wno-unused-function", "-Wno-unused-variable", "-Wno-unused-but-set-variable",
 ]
 
-HEADERS = ["fft_core.hpp", "kinds.hpp", "tile.hpp", "ktable.hpp", "engine.hpp"]
+HEADERS = ["fft_core.hpp", "kinds.hpp", "tile.hpp", "ktable.hpp", "engine.hpp", "big.hpp", "bigtable.hpp"]
 
 
 def _newer(target, deps):
@@ -49,6 +49,7 @@ def _units():
             o = os.path.join(OBJ, f"inst_f{64 if f64 else 32}_k{kind}.o")
             u.append((o, "inst.hip", [f"-DINST_F64={f64}", f"-DINST_KIND={kind}"], "engine"))
     u.append((os.path.join(OBJ, "twiddle.o"), "twiddle.hip", [], "engine"))
+    u.append((os.path.join(OBJ, "big_inst.o"), "big_inst.hip", [], "engine"))
     u.append((os.path.join(OBJ, "engine.o"), "engine.cpp", ["-x", "hip"], "engine"))
     u.append((os.path.join(OBJ, "batch.o"), "batch.cpp", ["-x", "hip"], "engine"))
     u.append((os.path.join(OBJ, "api64.o"), "api.cpp", ["-x", "hip"], "api64"))

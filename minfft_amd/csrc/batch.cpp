@@ -13,6 +13,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <tuple>
@@ -49,7 +50,17 @@ struct minfft_batch_plan {
     hipStream_t g_st = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    // branch streams/events for concurrent groups inside the captured graph
+    std::vector<hipStream_t> side;
+    std::vector<hipEvent_t> ev;
 };
+
+// groups run on up to this many concurrent graph branches
+static int branch_count() {
+    const char* e = std::getenv("MINFFT_BRANCHES");
+    int v = e ? std::atoi(e) : 4;
+    return v < 1 ? 1 : (v > 8 ? 8 : v);
+}
 
 static size_t item_in_bytes(int kind, Prec p) { return (kind_complex_in(kind) ? 2 : 1) * real_size(p); }
 static size_t item_out_bytes(int kind, Prec p) { return (kind_complex_out(kind) ? 2 : 1) * real_size(p); }
@@ -91,6 +102,8 @@ void minfft_free_batch(minfft_batch_plan* p) {
         if (g.tbuf) (void)hipFree(g.tbuf);
     }
     free_workspace(p->ws);
+    for (auto s : p->side) (void)hipStreamDestroy(s);
+    for (auto e : p->ev) (void)hipEventDestroy(e);
     if (p->stream) (void)hipStreamDestroy(p->stream);
     delete p;
 }
@@ -205,11 +218,60 @@ minfft_batch_plan* minfft_mkaux_batch(int count, const minfft_batch_entry* e, in
     return p;
 }
 
-static int enqueue_batch(minfft_batch_plan* p, const void* in, void* out, hipStream_t st) {
-    for (auto& g : p->groups) {
+static long long group_bytes(const BatchGroup& g, Prec pr) {
+    long long P = 1;
+    for (int k = 0; k < g.d; ++k) P *= g.n[k];
+    return P * (long long)g.ioff.size() * (long long)real_size(pr);
+}
+
+// Enqueue every group.  With branches > 1 the groups are spread (largest
+// first, onto the least-loaded branch) over side streams forked from and
+// joined back into `st`, so that in a captured graph they become parallel
+// branches: short groups fill the tails of the long ones.
+static int enqueue_batch(minfft_batch_plan* p, const void* in, void* out, hipStream_t st, int branches) {
+    const int nb = std::min<int>(branches, (int)p->groups.size());
+    if (nb <= 1) {
+        for (auto& g : p->groups) {
+            int rc = run_entries(p->ws, p->prec, g.kind, g.d, g.n, in, out, (unsigned)g.ioff.size(), g.d_ioff,
+                                 g.d_ooff, g.d_foff, g.tbuf, g.d_toff, st);
+            if (rc) return rc;
+        }
+        return 0;
+    }
+    while ((int)p->side.size() < nb) {
+        hipStream_t s;
+        if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) {
+            set_error("hipStreamCreate failed");
+            return -1;
+        }
+        p->side.push_back(s);
+    }
+    while ((int)p->ev.size() < nb + 1) {
+        hipEvent_t e;
+        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+            set_error("hipEventCreate failed");
+            return -1;
+        }
+        p->ev.push_back(e);
+    }
+    // fork
+    if (hipEventRecord(p->ev[nb], st) != hipSuccess) return -1;
+    for (int b = 0; b < nb; ++b)
+        if (hipStreamWaitEvent(p->side[b], p->ev[nb], 0) != hipSuccess) return -1;
+    std::vector<long long> load(nb, 0);
+    for (auto& g : p->groups) {  // groups are sorted largest first
+        int b = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        // the two-step scratch is shared: long groups stay on branch 0
+        if (g.fbytes) b = 0;
+        load[b] += group_bytes(g, p->prec);
         int rc = run_entries(p->ws, p->prec, g.kind, g.d, g.n, in, out, (unsigned)g.ioff.size(), g.d_ioff,
-                             g.d_ooff, g.d_foff, g.tbuf, g.d_toff, st);
+                             g.d_ooff, g.d_foff, g.tbuf, g.d_toff, p->side[b]);
         if (rc) return rc;
+    }
+    // join
+    for (int b = 0; b < nb; ++b) {
+        if (hipEventRecord(p->ev[b], p->side[b]) != hipSuccess) return -1;
+        if (hipStreamWaitEvent(st, p->ev[b], 0) != hipSuccess) return -1;
     }
     return 0;
 }
@@ -230,7 +292,7 @@ int minfft_batch_run(minfft_batch_plan* p, const void* in, void* out, void* stre
             set_error("hipStreamBeginCapture failed");
             return -1;
         }
-        int rc = enqueue_batch(p, in, out, st);
+        int rc = enqueue_batch(p, in, out, st, branch_count());
         hipGraph_t g = nullptr;
         hipError_t ec = hipStreamEndCapture(st, &g);
         if (rc || ec != hipSuccess) {
@@ -282,7 +344,7 @@ int minfft_batch(minfft_batch_plan* p, const void* in, size_t in_bytes, void* ou
     if (hipMemcpyAsync(di, in, in_bytes, hipMemcpyDefault, p->stream) != hipSuccess) rc = -1;
     // regions of `out` not covered by any entry keep their contents
     if (!rc && hipMemcpyAsync(dout_p, out, out_bytes, hipMemcpyDefault, p->stream) != hipSuccess) rc = -1;
-    if (!rc) rc = enqueue_batch(p, di, dout_p, p->stream);
+    if (!rc) rc = enqueue_batch(p, di, dout_p, p->stream, 1);
     if (!rc && hipMemcpyAsync(out, dout_p, out_bytes, hipMemcpyDefault, p->stream) != hipSuccess) rc = -1;
     if (hipStreamSynchronize(p->stream) != hipSuccess) rc = -1;
     (void)hipFree(di);

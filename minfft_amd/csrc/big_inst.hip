@@ -1,0 +1,40 @@
+// big_inst.hip -- instances of the fused two-step kernel (big.hpp) and their
+// registry: FP32 with 512-thread work-groups, FP64 with 256 (two work-groups
+// of ~70 KiB LDS per CU where the registers allow).
+#include "big.hpp"
+#include "bigtable.hpp"
+
+namespace mfx {
+
+template <typename T, int L1, int L2, int BT, int WPE>
+static void reg_one(BigTable& t, int prec, int variant = 0) {
+    const int lg = L1 + L2;
+    if (variant) {
+        t.alt[prec][lg][0] = reinterpret_cast<const void*>(&big_kernel<T, L1, L2, false, BT, WPE>);
+        t.alt[prec][lg][1] = reinterpret_cast<const void*>(&big_kernel<T, L1, L2, true, BT, WPE>);
+        t.alt_lds[prec][lg] = big_lds_bytes<T, L1, L2, BT>();
+        t.alt_bt[prec][lg] = BT;
+        t.alt_wpc[prec][lg] = (4 * WPE * 64) / BT;
+        return;
+    }
+    t.fn[prec][lg][0] = reinterpret_cast<const void*>(&big_kernel<T, L1, L2, false, BT, WPE>);
+    t.fn[prec][lg][1] = reinterpret_cast<const void*>(&big_kernel<T, L1, L2, true, BT, WPE>);
+    t.lds[prec][lg] = big_lds_bytes<T, L1, L2, BT>();
+    t.bt[prec][lg] = BT;
+    t.wpc[prec][lg] = (4 * WPE * 64) / BT;  // work-groups per CU the registers allow
+}
+
+void register_big(BigTable& t) {
+    // <T, log2 N1, log2 N2, threads, min waves per SIMD>
+    reg_one<float, 7, 7, 512, 2>(t, 1);
+    reg_one<float, 8, 7, 512, 4>(t, 1);
+    reg_one<float, 8, 8, 512, 4>(t, 1);
+    reg_one<double, 7, 7, 256, 2>(t, 0);
+    reg_one<double, 8, 7, 256, 2>(t, 0);
+    reg_one<double, 8, 8, 256, 2>(t, 0);
+    // smaller work-groups (4 per CU) for launch-shape comparison
+    reg_one<float, 8, 7, 256, 4>(t, 1, 1);
+    reg_one<float, 8, 8, 256, 4>(t, 1, 1);
+}
+
+}  // namespace mfx

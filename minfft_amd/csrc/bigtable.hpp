@@ -1,0 +1,21 @@
+// bigtable.hpp -- registry of the fused two-step kernel instances.
+#pragma once
+#include <cstddef>
+
+namespace mfx {
+
+struct BigTable {
+    const void* fn[2][17][2];  // [prec 0=f64 1=f32][log2 N][inverse]
+    size_t lds[2][17];
+    int bt[2][17];
+    int wpc[2][17];  // resident work-groups per CU
+    // alternative launch shape (smaller work-groups), where instantiated
+    const void* alt[2][17][2];
+    size_t alt_lds[2][17];
+    int alt_bt[2][17];
+    int alt_wpc[2][17];
+};
+
+void register_big(BigTable& t);
+
+}  // namespace mfx

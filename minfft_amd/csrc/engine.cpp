@@ -18,8 +18,10 @@
 #include <mutex>
 #include <string>
 
+#include "bigtable.hpp"
 #include "ktable.hpp"
 #include "tile.hpp"
+#include "big.hpp"
 
 namespace mfx {
 
@@ -69,6 +71,16 @@ void register_f32(KTable& t) {
 void register_f64(KTable& t) {
     register_f64_k0(t); register_f64_k1(t); register_f64_k2(t); register_f64_k3(t); register_f64_k4(t);
     register_f64_k5(t); register_f64_k6(t); register_f64_k7(t); register_f64_k8(t); register_f64_k9(t);
+}
+
+static BigTable g_big;
+static std::once_flag g_big_once;
+static const BigTable& bigtable() {
+    std::call_once(g_big_once, [] {
+        std::memset(&g_big, 0, sizeof(g_big));
+        register_big(g_big);
+    });
+    return g_big;
 }
 
 static KTable g_kt[2];
@@ -146,6 +158,10 @@ struct Tune {
     int cfb = 128;  // bytes per row segment of a c-fast tile
     int xcd = 1;
     int stage = 0;
+    int chunk_mb = 64;  // two-step chunk size for entry batches
+    int fused = 1;      // fused two-step kernel for batches of long 1D entries
+    int fused_wpc = 2;  // work-groups per CU of the fused kernel
+    int fused_alt = 0;  // use the alternative (smaller) fused work-groups
 };
 static Tune g_tune;
 static std::once_flag g_tune_once;
@@ -167,6 +183,10 @@ static const Tune& tune() {
                 else if (k == "cfb" && v > 0) g_tune.cfb = v;
                 else if (k == "xcd") g_tune.xcd = v;
                 else if (k == "stage") g_tune.stage = v;
+                else if (k == "chunk" && v > 0) g_tune.chunk_mb = v;
+                else if (k == "fused") g_tune.fused = v;
+                else if (k == "wpc" && v > 0) g_tune.fused_wpc = v;
+                else if (k == "falt") g_tune.fused_alt = v;
             }
             pos = c + 1;
         }
@@ -211,7 +231,8 @@ static void fill_levels(TileArgs& A, const Levels& lv, int* fs_level = nullptr) 
 }
 
 // One launch of the fused tile kernel.
-static int launch_tile(Prec prec, int kind, int logL, const void* tw, const void* in, long long in_e,
+static int launch_tile(Prec prec, int kind, int logL, const void* tw, const void* tw64, const void* in,
+                       long long in_e,
                        void* out, long long out_e, const Levels& lv, int fs_level, int fs_logN,
                        hipStream_t st) {
     const KTable& kt = ktable(prec);
@@ -229,6 +250,7 @@ static int launch_tile(Prec prec, int kind, int logL, const void* tw, const void
     A.in = in;
     A.out = out;
     A.tw = tw;
+    A.tw64 = (const double2*)tw64;
     A.in_e = in_e;
     A.out_e = out_e;
     fill_levels(A, lv, &fs_level);
@@ -325,7 +347,7 @@ static int c2c_axis(const Workspace& ws, Prec prec, int kind, int logL, const vo
                     void* out, long long out_e, const Levels& lv, hipStream_t st) {
     const int maxlog = device().maxlog[prec];
     if (logL <= maxlog)
-        return launch_tile(prec, kind, logL, ws.tw, in, in_e, out, out_e, lv, -1, 0, st);
+        return launch_tile(prec, kind, logL, ws.tw, ws.tw64, in, in_e, out, out_e, lv, -1, 0, st);
     if (in_e != out_e) {
         set_error("internal: two-step pass needs equal strides");
         return -1;
@@ -379,9 +401,72 @@ static int c2c_axis(const Workspace& ws, Prec prec, int kind, int logL, const vo
         set_error("internal: offset table level clash");
         return -1;
     }
-    int rc = launch_tile(prec, kind, l1, ws.tw, in, N2 * e, ws.fbuf, N2 * e, s1, fs_pos, logL, st);
-    if (rc) return rc;
-    return launch_tile(prec, kind, l2, ws.tw, ws.fbuf, e, out, N1 * e, s2, -1, 0, st);
+    // Batches of contiguous 1D entries: one fused launch, a work-group per
+    // transform, intermediate in a per-work-group scratch slot.
+    {
+        const BigTable& bt = bigtable();
+        const Tune& tn = tune();
+        const int pi = prec == PREC_F64 ? 0 : 1;
+        const int lg = std::min(logL, 16);
+        const bool alt = tn.fused_alt && bt.alt[pi][lg][0];
+        const void* fn = (logL <= 16) ? (alt ? bt.alt[pi][lg][kind == K_INVDFT] : bt.fn[pi][lg][kind == K_INVDFT])
+                                      : nullptr;
+        const int bwpc = alt ? bt.alt_wpc[pi][lg] : bt.wpc[pi][lg];
+        const unsigned bthreads = (unsigned)(alt ? bt.alt_bt[pi][lg] : bt.bt[pi][lg]);
+        const size_t blds = alt ? bt.alt_lds[pi][lg] : bt.lds[pi][lg];
+        const unsigned cnt = lv.n ? lv.B[0] : 1;
+        const int wpc = std::max(1, std::min(tn.fused_wpc, bwpc));
+        const unsigned grid = std::min<unsigned>(cnt, (unsigned)(device().ncu * wpc));
+        const size_t need = (size_t)grid * ((size_t)1 << logL) * (prec == PREC_F64 ? 16 : 8);
+        if (fn && tn.fused && lv.n == 1 && lv.ioff && lv.ooff && e == 1 && cnt >= (unsigned)device().ncu &&
+            need <= ws.fs_bytes) {
+            BigArgs B;
+            B.in = in;
+            B.out = out;
+            B.scratch = ws.fbuf;
+            B.tw = ws.tw;
+            B.tw64 = (const double2*)ws.tw64;
+            B.in_off = lv.ioff;
+            B.out_off = lv.ooff;
+            B.count = cnt;
+            void* args[] = {&B};
+            return launch_fn(fn, grid, bthreads, args, blds, st);
+        }
+    }
+    // With an entry table, run the two steps chunk by chunk (~64 MiB of
+    // data per chunk) so each chunk's intermediate is still in the MALL
+    // (infinity cache) when step 2 reads it back.
+    const size_t cs = prec == PREC_F64 ? 16 : 8;
+    const unsigned long long per_entry = lv.count() / (lv.n ? lv.B[0] : 1);
+    unsigned chunk = lv.n ? lv.B[0] : 1;
+    if ((lv.ioff || lv.ooff) && lv.n > 0) {
+        const size_t target = (size_t)tune().chunk_mb << 20;
+        const size_t ebytes = (size_t)per_entry * ((size_t)1 << logL) * cs;
+        chunk = (unsigned)std::max<size_t>(1, target / std::max<size_t>(ebytes, 1));
+        chunk = std::min(chunk, lv.B[0]);
+    }
+    const unsigned total = lv.n ? lv.B[0] : 1;
+    for (unsigned c0 = 0; c0 < total; c0 += chunk) {
+        const unsigned cn = std::min(chunk, total - c0);
+        Levels a = s1, b = s2;
+        if (lv.n && (lv.ioff || lv.ooff)) {
+            a.B[0] = b.B[0] = cn;
+            // scratch offsets: with a compact table every chunk reuses the
+            // same (first) scratch entries, keeping the footprint one chunk
+            const long long* fsc = lv.foff ? fo : (fo ? fo + c0 : nullptr);
+            a.ioff = s1.ioff ? s1.ioff + c0 : nullptr;
+            a.ooff = fsc;
+            b.ioff = fsc;
+            b.ooff = s2.ooff ? s2.ooff + c0 : nullptr;
+        } else if (c0 > 0) {
+            break;
+        }
+        int rc = launch_tile(prec, kind, l1, ws.tw, ws.tw64, in, N2 * e, ws.fbuf, N2 * e, a, fs_pos, logL, st);
+        if (rc) return rc;
+        rc = launch_tile(prec, kind, l2, ws.tw, ws.tw64, ws.fbuf, e, out, N1 * e, b, -1, 0, st);
+        if (rc) return rc;
+    }
+    return 0;
 }
 
 int axis_pass(const Workspace& ws, Prec prec, int kind, int logN, const void* in, long long in_e,
@@ -406,7 +491,7 @@ int axis_pass(const Workspace& ws, Prec prec, int kind, int logN, const void* in
     }
     const int logL = logN - 1;
     if (logL <= maxlog)
-        return launch_tile(prec, kind, logL, ws.tw, in, in_e, out, out_e, lv, -1, 0, st);
+        return launch_tile(prec, kind, logL, ws.tw, ws.tw64, in, in_e, out, out_e, lv, -1, 0, st);
     // long real transform: pre-kernel -> two-step complex FFT -> post-kernel
     if (!ws.gbuf || !ws.fbuf) {
         set_error("internal: missing two-step scratch");

@@ -48,6 +48,15 @@ template <typename T> __device__ __forceinline__ c2<T> mul_pi(c2<T> a) { return 
 
 constexpr int ilog2(int x) { return x <= 1 ? 0 : 1 + ilog2(x >> 1); }
 
+// Returns x, but hides its value from the optimiser (an empty asm statement
+// on a register: no instruction, no memory access).  Used to keep loop-
+// invariant twiddle loads inside the loops of the persistent kernels, where
+// hoisting them would pin dozens of registers.
+__device__ __forceinline__ int opaque(int x) {
+    asm volatile("" : "+v"(x));
+    return x;
+}
+
 // cos(2*pi*k/32), k = 0..8
 __device__ constexpr double kCos32[9] = {
     1.0,
@@ -167,9 +176,10 @@ struct Core {
         __device__ __forceinline__ void load(int tl, const c2<T>* __restrict__ tw) {
             if constexpr (Ns > 1) {
                 const c2<T>* twr = tw + Ns * R;
+                const int tlo = opaque(tl);
 #pragma unroll
                 for (int b = 0; b < NB; ++b) {
-                    const int k = (tl + b * TPT) & (Ns - 1);
+                    const int k = (tlo + b * TPT) & (Ns - 1);
 #pragma unroll
                     for (int r = 1; r < R; ++r) w[b * (R - 1) + r - 1] = twr[r * k];
                 }

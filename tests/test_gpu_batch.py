@@ -1,0 +1,146 @@
+"""GPU parity of the batch extension (minfft_mkaux_batch / minfft_batch_run):
+every entry must equal the single-transform routine applied to it, i.e.
+the CPU oracle, to the same bars as the plain API."""
+import numpy as np
+import pytest
+
+import minfft_amd as mf
+from oracle.oracle import in_shape, out_shape, is_complex_in, is_complex_out
+from tests.helpers import TOL32, make_input, rel_l2, tol64
+
+pytestmark = pytest.mark.gpu
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def run_batch(entries, prec, xs, oracle64, check_idx=None, seed=0):
+    """entries: list of (kind, shape); xs: list of numpy inputs.  Packs the
+    inputs into one device buffer with gaps, runs the batch, checks entries."""
+    torch = _torch()
+    rdt = np.float64 if prec == 64 else np.float32
+    cdt = np.complex128 if prec == 64 else np.complex64
+    rs = np.dtype(rdt).itemsize
+    ioffs, ooffs = [], []
+    ipos = opos = 0
+    for (kind, shape), x in zip(entries, xs):
+        ioffs.append(ipos)
+        ipos += x.nbytes + 64 * rs  # gap between entries
+        n_out = int(np.prod(out_shape(kind, shape))) * (2 if is_complex_out(kind) else 1)
+        ooffs.append(opos)
+        opos += n_out * rs + 32 * rs
+    hin = np.zeros(ipos, dtype=np.uint8)
+    for x, o in zip(xs, ioffs):
+        hin[o:o + x.nbytes] = np.frombuffer(np.ascontiguousarray(x).tobytes(), dtype=np.uint8)
+    din = torch.from_numpy(hin).cuda()
+    dout = torch.zeros(opos, dtype=torch.uint8, device="cuda")
+    b = mf.Batch([(k, s, io, oo) for (k, s), io, oo in zip(entries, ioffs, ooffs)],
+                 mf.PREC_FP64 if prec == 64 else mf.PREC_FP32)
+    st = torch.cuda.Stream()
+    b.run(din.data_ptr(), dout.data_ptr(), st.cuda_stream)
+    torch.cuda.synchronize()
+    b.run(din.data_ptr(), dout.data_ptr(), st.cuda_stream)  # graph replay
+    torch.cuda.synchronize()
+    hout = dout.cpu().numpy()
+    b.close()
+    idx = range(len(entries)) if check_idx is None else check_idx
+    worst = 0.0
+    for i in idx:
+        kind, shape = entries[i]
+        odt = cdt if is_complex_out(kind) else rdt
+        n_out = int(np.prod(out_shape(kind, shape)))
+        y = hout[ooffs[i]:ooffs[i] + n_out * np.dtype(odt).itemsize].view(odt).reshape(out_shape(kind, shape))
+        x = xs[i]
+        ref = oracle64.run(kind, x.astype(np.complex128 if is_complex_in(kind) else np.float64), shape)
+        e = rel_l2(y, ref)
+        bar = tol64(shape) if prec == 64 else TOL32
+        assert e <= bar, (i, kind, shape, prec, e)
+        worst = max(worst, e)
+    # the gaps between outputs stay untouched
+    mask = np.ones(opos, dtype=bool)
+    for (kind, shape), oo in zip(entries, ooffs):
+        n_out = int(np.prod(out_shape(kind, shape))) * (2 if is_complex_out(kind) else 1)
+        mask[oo:oo + n_out * rs] = False
+    assert not hout[mask].any()
+    return worst
+
+
+@pytest.mark.parametrize("prec", [32, 64])
+def test_uneven_1d_c2c_batch(oracle64, prec):
+    """Mixed lengths 2^0..2^17, forward and inverse, small counts (tile and
+    chunked two-step paths)."""
+    rng = np.random.default_rng(5)
+    entries, xs = [], []
+    for k in list(range(0, 18)) * 2:
+        kind = "dft" if rng.random() < 0.6 else "invdft"
+        entries.append((kind, (1 << k,)))
+        xs.append(make_input(kind, (1 << k,), prec, int(rng.integers(1 << 30))))
+    print(f"[case] uneven 1D batch prec={prec} entries={len(entries)}", flush=True)
+    run_batch(entries, prec, xs, oracle64)
+
+
+@pytest.mark.parametrize("prec", [32, 64])
+@pytest.mark.parametrize("logn", [14, 15, 16])
+def test_fused_long_batch(oracle64, prec, logn):
+    """>= 256 entries of one long length: the fused two-step kernel (one
+    work-group per transform, private scratch slot)."""
+    n = 1 << logn
+    cnt = 300
+    rng = np.random.default_rng(logn)
+    entries, xs = [], []
+    for i in range(cnt):
+        kind = "dft" if i % 3 else "invdft"
+        entries.append((kind, (n,)))
+        xs.append(make_input(kind, (n,), prec, int(rng.integers(1 << 30))))
+    print(f"[case] fused batch prec={prec} N=2^{logn} count={cnt}", flush=True)
+    # every entry computed; check a spread including first/last of each
+    # work-group round (static striding over 2*256 work-groups)
+    idx = sorted(set([0, 1, 2, 255, 256, 257, cnt - 2, cnt - 1] + list(range(0, cnt, 37))))
+    run_batch(entries, prec, xs, oracle64, check_idx=idx)
+
+
+@pytest.mark.parametrize("prec", [32, 64])
+def test_mixed_real_kinds_2d_batch(oracle64, prec):
+    rng = np.random.default_rng(11)
+    kinds = ["dct2", "dst2", "dct4", "dst4", "dct3", "dst3", "realdft", "invrealdft"]
+    shapes = [(64, 64), (16, 32), (8, 8, 4), (128, 2), (1, 16), (32,)]
+    entries, xs = [], []
+    for i in range(40):
+        kind = kinds[i % len(kinds)]
+        shape = shapes[(i // len(kinds) + i) % len(shapes)]
+        if kind == "invrealdft":
+            xr = make_input("dct2", shape, prec, 1000 + i)
+            x = oracle64.run("realdft", xr.astype(np.float64), shape).astype(
+                np.complex128 if prec == 64 else np.complex64)
+        else:
+            x = make_input(kind, shape, prec, 1000 + i)
+        entries.append((kind, shape))
+        xs.append(x)
+    print(f"[case] mixed real kinds prec={prec}", flush=True)
+    run_batch(entries, prec, xs, oracle64)
+
+
+def test_batch_host_pointers(oracle64):
+    """minfft_batch: host buffers, synchronous."""
+    sizes = [64, 1024, 4096, 256]
+    esz = 8
+    offs = np.concatenate([[0], np.cumsum(sizes)[:-1]])
+    x = make_input("dft", (int(sum(sizes)),), 32, 3)
+    y = np.zeros_like(x)
+    b = mf.Batch([("dft", (n,), int(o) * esz, int(o) * esz) for n, o in zip(sizes, offs)], mf.PREC_FP32)
+    b.run_sync(x, y)
+    b.close()
+    for n, o in zip(sizes, offs):
+        ref = oracle64.run("dft", x[o:o + n].astype(np.complex128), (n,))
+        assert rel_l2(y[o:o + n], ref) <= TOL32
+
+
+def test_batch_rejects_bad_entries():
+    with pytest.raises(mf.MinfftError):
+        mf.Batch([("dft", (12,), 0, 0)], mf.PREC_FP32)
+    with pytest.raises(mf.MinfftError):
+        mf.Batch([("dft", (16,), 4, 0)], mf.PREC_FP32)  # misaligned offset
+    with pytest.raises(mf.MinfftError):
+        mf.Batch([], mf.PREC_FP32)
