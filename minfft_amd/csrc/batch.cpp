@@ -209,6 +209,15 @@ minfft_batch_plan* minfft_mkaux_batch(int count, const minfft_batch_entry* e, in
         return pa * (long long)a.ioff.size() > pb * (long long)b.ioff.size();
     });
     if (alloc_twiddles(p->ws, p->prec, twlog, p->stream)) return fail(nullptr);
+    {
+        // work-queue counters of the cooperative kernel (largest 1D group)
+        size_t mx = 0;
+        for (auto& g : p->groups)
+            if (g.d == 1) mx = std::max(mx, g.ioff.size());
+        p->ws.nctrs = 8 + 2 * mx;
+        if (hipMalloc(&p->ws.ctrs, p->ws.nctrs * sizeof(unsigned)) != hipSuccess)
+            return fail("hipMalloc batch counters failed");
+    }
     if (fs_max) {
         if (hipMalloc(&p->ws.fbuf, fs_max) != hipSuccess) return fail("hipMalloc batch scratch failed");
         if (hipMalloc(&p->ws.gbuf, fs_max) != hipSuccess) return fail("hipMalloc batch scratch failed");

@@ -14,6 +14,12 @@ struct BigTable {
     size_t alt_lds[2][17];
     int alt_bt[2][17];
     int alt_wpc[2][17];
+    // XCD-cooperative kernels (coop.hpp), two launch-shape variants
+    const void* coop[2][17][2][2];
+    size_t coop_lds[2][17][2];
+    int coop_bt[2][17][2];
+    int coop_wpc[2][17][2];
+    const void* coop_reset;
 };
 
 void register_big(BigTable& t);

@@ -56,6 +56,8 @@ struct Workspace {
     void* fbuf = nullptr;      // two-step intermediate
     void* gbuf = nullptr;      // two-step complex staging for real kinds
     size_t fs_bytes = 0;
+    unsigned* ctrs = nullptr;  // work-queue counters of the cooperative kernel
+    size_t nctrs = 0;
 };
 
 struct Plan {
