@@ -138,6 +138,259 @@ def cpu_baseline(sizes, budget_s=12.0):
                       f"FP32 {'-O3 -march=x86-64-v3' if fast else '-O2'}, one transform at a time"}
 
 
+# --------------------------------------------------------------------------
+# other configurations (printed with --config; the default line is C3)
+# --------------------------------------------------------------------------
+def _events(torch, st):
+    return torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+
+
+def _time_enqueue(torch, fn, steps, warmup, st):
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    e0, e1 = _events(torch, st)
+    e0.record(st)
+    for _ in range(steps):
+        fn()
+    e1.record(st)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / steps
+
+
+def _line(metric, value, unit, ms, steps, warmup, dtype, workload, extra, higher=True):
+    out = {"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": steps, "warmup": warmup,
+           "ms_per_step": round(ms, 5), "higher_is_better": higher, "scaling": "weak", "vs_baseline": None,
+           "dtype": dtype, "data": "synthetic", "config": {"workload": workload, "parallelism": "single GPU"}}
+    out.update(extra)
+    return out
+
+
+def config_c1(a, torch, mf):
+    """1D C2C N=1024 FP64, dft then invdft, single transform."""
+    from oracle.oracle import Oracle, Reference, ref_available
+    N = 1024
+    rng = np.random.default_rng(1)
+    x = (rng.uniform(-1, 1, N) + 1j * rng.uniform(-1, 1, N)).astype(np.complex128)
+    lib = mf.Minfft(64)
+    aux = mf.Aux(lib, "dft", (N,))
+    st = torch.cuda.Stream()
+    xd = torch.from_numpy(x).cuda()
+    yd = torch.empty_like(xd)
+    bd = torch.empty_like(xd)
+    sh = st.cuda_stream
+
+    def step():
+        aux.enqueue("dft", xd, yd, sh)
+        aux.enqueue("invdft", yd, bd, sh)
+    ms = _time_enqueue(torch, step, a.steps * 50, a.warmup, st)
+    # host-pointer (reference-style) latency
+    y = np.empty_like(x)
+    b = np.empty_like(x)
+    t0 = time.perf_counter()
+    reps = 200
+    for _ in range(reps):
+        aux("dft", x, y)
+        aux("invdft", y, b)
+    host_us = (time.perf_counter() - t0) / reps * 1e6
+    o = Oracle(64)
+    ref = o.run("dft", x, (N,))
+    yg = yd.cpu().numpy()
+    rel = float(np.linalg.norm(yg - ref) / np.linalg.norm(ref))
+    rt = float(np.linalg.norm(bd.cpu().numpy() / N - x) / np.linalg.norm(x))
+    relref = None
+    if ref_available(64):
+        r = Reference(64).run("dft", x, (N,))
+        relref = float(np.linalg.norm(yg - r) / np.linalg.norm(r))
+    cpu = None
+    if ref_available(64, fast=True):
+        R = Reference(64, fast=True)
+        ra = R.mkaux("dft", (N,))
+        f = R.fn("dft")
+        fi = R.fn("invdft")
+        yy = np.empty_like(x)
+        bb = np.empty_like(x)
+        reps = 20000
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            f(x.ctypes.data, yy.ctypes.data, ra)
+            fi(yy.ctypes.data, bb.ctypes.data, ra)
+        dt = (time.perf_counter() - t0) / reps
+        R.free(ra)
+        cpu = {"value": round(dt * 1e6, 3), "unit": "us per dft+invdft pair", "cores": 1, "kind": "reference",
+               "sample": f"{reps} round trips of N=1024 through ctypes"}
+    aux.close()
+    flops = 2 * 5.0 * N * 10
+    return _line("1D C2C FP64 dft+invdft latency", round(ms * 1000, 3), "us per dft+invdft pair (device)", ms,
+                 a.steps * 50, a.warmup, "f64", "C1: N=1024 C2C FP64 round trip",
+                 {"gflops": round(flops / (ms * 1e-3) / 1e9, 3), "host_pointer_us": round(host_us, 2),
+                  "rel_l2_vs_oracle": rel, "rel_l2_vs_reference": relref, "round_trip_rel_l2": rt,
+                  "tolerance": 1e-14 * 10, "cpu_baseline": cpu}, higher=False)
+
+
+def config_c2(a, torch, mf):
+    """2D real DFT and DCT-2/DCT-3 on 2048x2048 FP64."""
+    from oracle.oracle import Oracle, Reference, ref_available
+    n = 2048
+    shape = (n, n)
+    g = torch.Generator(device="cuda")
+    g.manual_seed(2)
+    xd = torch.randn(shape, generator=g, device="cuda", dtype=torch.float64)
+    zd = torch.empty((n, n // 2 + 1), dtype=torch.complex128, device="cuda")
+    yd = torch.empty_like(xd)
+    cd = torch.empty_like(xd)
+    dd = torch.empty_like(xd)
+    lib = mf.Minfft(64)
+    ar = mf.Aux(lib, "realdft", shape)
+    at = mf.Aux(lib, "t2t3", shape)
+    st = torch.cuda.Stream()
+    sh = st.cuda_stream
+    res = {}
+    for name, fn in (("realdft", lambda: ar.enqueue("realdft", xd, zd, sh)),
+                     ("invrealdft", lambda: ar.enqueue("invrealdft", zd, yd, sh)),
+                     ("dct2", lambda: at.enqueue("dct2", xd, cd, sh)),
+                     ("dct3", lambda: at.enqueue("dct3", cd, dd, sh))):
+        res[name] = _time_enqueue(torch, fn, a.steps, a.warmup, st)
+    o = Oracle(64)
+    x = xd.cpu().numpy()
+    t0 = time.perf_counter()
+    zr = o.run("realdft", x, shape)
+    cr = o.run("dct2", x, shape)
+    oracle_s = time.perf_counter() - t0
+    N = n * n
+    tol = 1e-14 * 22
+    err = {"realdft": float(np.linalg.norm(zd.cpu().numpy() - zr) / np.linalg.norm(zr)),
+           "dct2": float(np.linalg.norm(cd.cpu().numpy() - cr) / np.linalg.norm(cr)),
+           "invrealdft_round_trip": float(np.linalg.norm(yd.cpu().numpy() / N - x) / np.linalg.norm(x)),
+           "dct3_of_dct2": float(np.linalg.norm(dd.cpu().numpy() / (4 * N) - x) / np.linalg.norm(x))}
+    cpu = None
+    if ref_available(64, fast=True):
+        R = Reference(64, fast=True)
+        ra = R.mkaux("realdft", shape)
+        zz = np.empty((n, n // 2 + 1), dtype=np.complex128)
+        t0 = time.perf_counter()
+        R.fn("realdft")(x.ctypes.data, zz.ctypes.data, ra)
+        dt = time.perf_counter() - t0
+        R.free(ra)
+        cpu = {"value": round(2.5 * N * 22 / dt / 1e9, 3), "unit": "GFLOP/s (2.5 N log2 N)", "cores": 1,
+               "kind": "reference", "sample": f"one 2048x2048 realdft, {dt*1e3:.1f} ms"}
+    alg = {"realdft": N * 8 + n * (n // 2 + 1) * 16, "invrealdft": N * 8 + n * (n // 2 + 1) * 16,
+           "dct2": 2 * N * 8, "dct3": 2 * N * 8}
+    flops = 2.5 * N * 22
+    per = {k: {"ms": round(v, 5), "gflops": round(flops / (v * 1e-3) / 1e9, 2),
+               "alg_hbm_gbs": round(alg[k] / (v * 1e-3) / 1e9, 1)} for k, v in res.items()}
+    ms = res["realdft"]
+    return _line("2D real DFT 2048x2048 FP64 GFLOP/s", round(flops / (ms * 1e-3) / 1e9, 3), "GFLOP/s", ms,
+                 a.steps, a.warmup, "f64", "C2: 2048x2048 realdft/invrealdft + DCT-2/DCT-3, FP64",
+                 {"per_transform": per, "rel_l2": err, "tolerance": tol, "oracle_seconds": round(oracle_s, 2),
+                  "roofline": {"bound": "hbm", "achieved": per["realdft"]["alg_hbm_gbs"], "peak": HBM_PEAK_GBS,
+                               "unit": "GB/s", "frac": round(per["realdft"]["alg_hbm_gbs"] / HBM_PEAK_GBS, 4),
+                               "traffic": None, "kernel": "realdft (2 axis passes)"},
+                  "cpu_baseline": cpu})
+
+
+def config_c4(a, torch, mf):
+    """3D C2C 512^3 complex64 (FP32 path) vs the FP64 path on the same input."""
+    n = 512
+    shape = (n, n, n)
+    rng = np.random.default_rng(4)
+    ks = set()
+    while len(ks) < 64:
+        ks.add(tuple(int(v) for v in rng.integers(0, n, 3)))
+    ks = sorted(ks)
+    phis = rng.uniform(0, 2 * np.pi, 64)
+    # the plane-wave field = unnormalised inverse DFT of its 64-spike spectrum
+    spec = torch.zeros(shape, dtype=torch.complex128, device="cuda")
+    for (kx, ky, kz), p in zip(ks, phis):
+        spec[kx, ky, kz] = complex(np.cos(p), np.sin(p))
+    lib64 = mf.Minfft(64)
+    lib32 = mf.Minfft(32)
+    a64 = mf.Aux(lib64, "dft", shape)
+    st = torch.cuda.Stream()
+    sh = st.cuda_stream
+    field = torch.empty_like(spec)
+    a64.enqueue("invdft", spec, field, sh)
+    torch.cuda.synchronize()
+    del spec
+    g = torch.Generator(device="cuda")
+    g.manual_seed(4)
+    field += torch.complex(torch.randn(shape, generator=g, device="cuda", dtype=torch.float64),
+                           torch.randn(shape, generator=g, device="cuda", dtype=torch.float64)) * 1e-3
+    x32 = field.to(torch.complex64)
+    del field
+    torch.cuda.empty_cache()
+    y32 = torch.empty_like(x32)
+    a32 = mf.Aux(lib32, "dft", shape)
+    ms32 = _time_enqueue(torch, lambda: a32.enqueue("dft", x32, y32, sh), a.steps, a.warmup, st)
+    x64 = x32.to(torch.complex128)
+    y64 = torch.empty_like(x64)
+    ms64 = _time_enqueue(torch, lambda: a64.enqueue("dft", x64, y64, sh), max(3, a.steps // 2), 1, st)
+    d = (y32.to(torch.complex128) - y64)
+    rel = float(torch.linalg.vector_norm(d) / torch.linalg.vector_norm(y64))
+    mx = float(d.abs().max())
+    # closed form at the spikes: 512^3 e^{i phi}
+    spikes = torch.stack([y64[kx, ky, kz] for (kx, ky, kz) in ks]).cpu().numpy()
+    want = (n ** 3) * np.exp(1j * phis)
+    spike_rel = float(np.max(np.abs(spikes - want)) / n ** 3)
+    N = n ** 3
+    alg = 2 * N * 8
+    flops = 5.0 * N * 27
+    a32.close()
+    a64.close()
+    return _line("3D C2C 512^3 complex64 GFLOP/s", round(flops / (ms32 * 1e-3) / 1e9, 3), "GFLOP/s", ms32,
+                 a.steps, a.warmup, "f32", "C4: 512^3 C2C, 64 plane waves + N(0,1e-3) noise; FP32 passes",
+                 {"fp64_gpu_ms": round(ms64, 4), "speedup_vs_fp64_gpu": round(ms64 / ms32, 3),
+                  "rel_l2_vs_fp64": rel, "max_abs_vs_fp64": mx, "spike_closed_form_rel": spike_rel,
+                  "roofline": {"bound": "hbm", "achieved": round(alg / (ms32 * 1e-3) / 1e9, 1),
+                               "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                               "frac": round(alg / (ms32 * 1e-3) / 1e9 / HBM_PEAK_GBS, 4), "traffic": None,
+                               "kernel": "3 axis passes"},
+                  "note": "split-FP16 MFMA radix-16 stage not built yet: FP32 butterflies"})
+
+
+def config_c5(a, torch, mf):
+    """256 images 1024x1024 FP32, DCT-2 / DST-2 / DCT-4 by image index mod 3, one batch."""
+    from oracle.oracle import Oracle
+    n, count = 1024, 256
+    kinds = ("dct2", "dst2", "dct4")
+    g = torch.Generator(device="cuda")
+    g.manual_seed(5)
+    m = torch.rand((count, n // 8, n // 8), generator=g, device="cuda", dtype=torch.float32)
+    x = m.repeat_interleave(8, 1).repeat_interleave(8, 2)
+    x += (torch.rand((count, n, n), generator=g, device="cuda", dtype=torch.float32) - 0.5) * 0.1
+    x = x.contiguous()
+    y = torch.empty_like(x)
+    img = n * n * 4
+    plan = mf.Batch([(kinds[i % 3], (n, n), i * img, i * img) for i in range(count)], mf.PREC_FP32)
+    st = torch.cuda.Stream()
+    sh = st.cuda_stream
+    ms = _time_enqueue(torch, lambda: plan.run(x.data_ptr(), y.data_ptr(), sh), a.steps, a.warmup, st)
+    plan.close()
+    o = Oracle(64)
+    errs = {}
+    for i in range(3):
+        xi = x[i].cpu().numpy().astype(np.float64)
+        r = o.run(kinds[i], xi, (n, n))
+        errs[kinds[i]] = float(np.linalg.norm(y[i].cpu().numpy() - r) / np.linalg.norm(r))
+    # FP64 batch on the same workload
+    x64 = x.to(torch.float64)
+    y64 = torch.empty_like(x64)
+    p64 = mf.Batch([(kinds[i % 3], (n, n), i * img * 2, i * img * 2) for i in range(count)], mf.PREC_FP64)
+    ms64 = _time_enqueue(torch, lambda: p64.run(x64.data_ptr(), y64.data_ptr(), sh), max(3, a.steps // 2), 1, st)
+    p64.close()
+    N = n * n * count
+    alg = 2 * N * 4
+    flops = 2.5 * N * 20
+    return _line("batched 2D DCT-2/DST-2/DCT-4 1024x1024 FP32 GFLOP/s", round(flops / (ms * 1e-3) / 1e9, 3),
+                 "GFLOP/s", ms, a.steps, a.warmup, "f32",
+                 "C5: 256 x 1024^2 FP32, type = (DCT-2, DST-2, DCT-4)[i mod 3], one batch",
+                 {"rel_l2_vs_fp64_oracle": errs, "fp64_gpu_ms": round(ms64, 4),
+                  "speedup_vs_fp64_gpu": round(ms64 / ms, 3),
+                  "roofline": {"bound": "hbm", "achieved": round(alg / (ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,
+                               "unit": "GB/s", "frac": round(alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                               "traffic": None, "kernel": "2 axis passes per image"}})
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -145,6 +398,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison leg")
+    ap.add_argument("--config", default="c3", choices=["c1", "c2", "c3", "c4", "c5"],
+                    help="c3 (default) is the metric's workload; the others print their own line")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the batched transforms are independent, so "
@@ -153,6 +408,11 @@ def main():
         sys.exit(2)
     import torch
     import minfft_amd as mf
+
+    if a.config != "c3":
+        fn = {"c1": config_c1, "c2": config_c2, "c4": config_c4, "c5": config_c5}[a.config]
+        print(json.dumps(fn(a, torch, mf)), flush=True)
+        return
 
     sizes = workload()
     total_pts = int(sizes.sum())

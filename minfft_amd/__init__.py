@@ -154,6 +154,14 @@ class Aux:
         self.lib.run_ptr(kind, _ptr(x), _ptr(y), self.h)
         return y
 
+    def enqueue(self, kind, x, y, stream=None):
+        """Asynchronous run on device pointers (minfft_enqueue, minfft_ext.h)."""
+        assert FAMILY[kind] == self.family
+        rc = engine().minfft_enqueue(KIND_ID[kind], _ptr(x), _ptr(y), self.h, stream)
+        if rc:
+            raise MinfftError(f"minfft_enqueue({kind}): {last_error()}")
+        return y
+
     def close(self):
         if self.h:
             self.lib.free_aux(self.h)
