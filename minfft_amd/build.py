@@ -31,7 +31,7 @@ COMMON = [
     "-Wall", "-Wno-unused-function", "-Wno-unused-variable", "-Wno-unused-but-set-variable",
 ]
 
-HEADERS = ["fft_core.hpp", "kinds.hpp", "tile.hpp", "ktable.hpp", "engine.hpp", "big.hpp", "bigtable.hpp", "coop.hpp"]
+HEADERS = ["fft_core.hpp", "kinds.hpp", "tile.hpp", "ktable.hpp", "engine.hpp", "big.hpp", "bigtable.hpp", "coop.hpp", "mfma.hpp"]
 
 
 def _newer(target, deps):

@@ -114,13 +114,13 @@ minfft_batch_plan* minfft_mkaux_batch(int count, const minfft_batch_entry* e, in
         set_error("empty batch");
         return nullptr;
     }
-    if (prec != MINFFT_FP64 && prec != MINFFT_FP32) {
-        set_error("precision not supported by this batch path");
+    if (prec != MINFFT_FP64 && prec != MINFFT_FP32 && prec != MINFFT_SPLIT_FP16) {
+        set_error("unknown precision");
         return nullptr;
     }
     if (!ensure_device()) return nullptr;
     minfft_batch_plan* p = new minfft_batch_plan();
-    p->prec = prec == MINFFT_FP64 ? PREC_F64 : PREC_F32;
+    p->prec = prec == MINFFT_FP64 ? PREC_F64 : PREC_F32;  // split-FP16 keeps FP32 data
     p->prec_req = prec;
     auto fail = [&](const char* msg) -> minfft_batch_plan* {
         if (msg) set_error(msg);
@@ -209,6 +209,7 @@ minfft_batch_plan* minfft_mkaux_batch(int count, const minfft_batch_entry* e, in
         return pa * (long long)a.ioff.size() > pb * (long long)b.ioff.size();
     });
     if (alloc_twiddles(p->ws, p->prec, twlog, p->stream)) return fail(nullptr);
+    p->ws.split16 = (prec == MINFFT_SPLIT_FP16) ? 1 : 0;
     {
         // work-queue counters of the cooperative kernel (largest 1D group)
         size_t mx = 0;

@@ -3,6 +3,7 @@
 // of ~70 KiB LDS per CU where the registers allow).
 #include "big.hpp"
 #include "coop.hpp"
+#include "mfma.hpp"
 #include "bigtable.hpp"
 
 namespace mfx {
@@ -36,6 +37,8 @@ static void reg_coop(BigTable& t, int prec, int v) {
 }
 
 void register_big(BigTable& t) {
+    t.mfma512[0] = reinterpret_cast<const void*>(&mfma512_kernel<false>);
+    t.mfma512[1] = reinterpret_cast<const void*>(&mfma512_kernel<true>);
     t.coop_reset = reinterpret_cast<const void*>(&coop_reset);
     // XCD-cooperative instances: variant 0 = 256 threads, 1 = 512 threads
     reg_coop<float, 8, 8, 256, 4>(t, 1, 0);

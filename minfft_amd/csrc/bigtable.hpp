@@ -20,6 +20,8 @@ struct BigTable {
     int coop_bt[2][17][2];
     int coop_wpc[2][17][2];
     const void* coop_reset;
+    // 512-point FP32 tiles with the split-FP16 MFMA radix-16 stage (mfma.hpp)
+    const void* mfma512[2];
 };
 
 void register_big(BigTable& t);

@@ -246,15 +246,16 @@ static void fill_levels(TileArgs& A, const Levels& lv, int* fs_level = nullptr) 
 static int launch_tile(Prec prec, int kind, int logL, const void* tw, const void* tw64, const void* in,
                        long long in_e,
                        void* out, long long out_e, const Levels& lv, int fs_level, int fs_logN,
-                       hipStream_t st) {
+                       hipStream_t st, bool split16 = false) {
     const KTable& kt = ktable(prec);
     Device& D = device();
     if (logL > D.maxlog[prec]) {
         set_error("internal: tile length beyond the single-tile limit");
         return -1;
     }
-    const void* fn = kt.tile[kind][logL];
-    const int E = kt.tile_E[logL];
+    split16 = split16 && prec == PREC_F32 && logL == 9 && (kind == K_DFT || kind == K_INVDFT) && fs_logN == 0;
+    const void* fn = split16 ? bigtable().mfma512[kind == K_INVDFT] : kt.tile[kind][logL];
+    const int E = split16 ? 16 : kt.tile_E[logL];
     const int L = 1 << logL;
     const int TPT = L / E;
     TileArgs A;
@@ -313,6 +314,7 @@ static int launch_tile(Prec prec, int kind, int logL, const void* tw, const void
         C = std::max(1, std::min(tn.tpw, maxT) / TPT);
     }
     if ((unsigned long long)C > ntr) C = (int)ntr;
+    if (split16) C = 16;  // the MFMA tile is laid out for 16 transforms, 512 threads
     size_t budget = (size_t)D.lds_max;
     while (C > 1 && (size_t)C * sc * cs > budget) C >>= 1;
     if ((size_t)C * sc * cs > budget) {
@@ -359,7 +361,7 @@ static int c2c_axis(const Workspace& ws, Prec prec, int kind, int logL, const vo
                     void* out, long long out_e, const Levels& lv, hipStream_t st) {
     const int maxlog = device().maxlog[prec];
     if (logL <= maxlog)
-        return launch_tile(prec, kind, logL, ws.tw, ws.tw64, in, in_e, out, out_e, lv, -1, 0, st);
+        return launch_tile(prec, kind, logL, ws.tw, ws.tw64, in, in_e, out, out_e, lv, -1, 0, st, ws.split16 != 0);
     if (in_e != out_e) {
         set_error("internal: two-step pass needs equal strides");
         return -1;

@@ -58,6 +58,7 @@ struct Workspace {
     size_t fs_bytes = 0;
     unsigned* ctrs = nullptr;  // work-queue counters of the cooperative kernel
     size_t nctrs = 0;
+    int split16 = 0;           // FP32 data, radix-16 stages as split-FP16 MFMA products
 };
 
 struct Plan {

@@ -268,6 +268,41 @@ struct Core {
     }
 };
 
+// One Stockham pass of radix R (stride Ns) of a length-L transform held as E
+// owned points per thread (TPT = L/E threads), outside the Core recursion:
+// v holds the owned positions on entry; a non-last pass scatters its results
+// into the LDS slot, the last pass leaves them in v at the owned positions.
+template <typename T, int L, int E, int R, int Ns, bool INV, bool LAST>
+__device__ __forceinline__ void stockham_pass(c2<T> (&v)[E], c2<T>* slot, int tl, const c2<T>* __restrict__ tw) {
+    constexpr int TPT = L / E;
+    constexpr int NB = E / R;
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+        const int j = tl + b * TPT;
+        const int k = j & (Ns - 1);
+        c2<T> a[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) a[r] = v[b + r * NB];
+        if constexpr (Ns > 1) {
+            const c2<T>* twr = tw + Ns * R;
+#pragma unroll
+            for (int r = 1; r < R; ++r) {
+                const c2<T> w = twr[r * k];
+                a[r] = INV ? cmulc<T>(a[r], w) : cmul<T>(a[r], w);
+            }
+        }
+        dft<R, INV, T>(a);
+        if constexpr (LAST) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[b + r * NB] = a[r];
+        } else {
+            const int base = (j - k) * R + k;
+#pragma unroll
+            for (int r = 0; r < R; ++r) slot[lpad(base + r * Ns)] = a[r];
+        }
+    }
+}
+
 // Points per thread used for a given length: radix-16 passes (E = 16) for
 // L >= 16, one pass of radix L below.
 constexpr int default_E(int logL) { return logL >= 4 ? 16 : (1 << logL); }

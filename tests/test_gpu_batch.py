@@ -16,7 +16,7 @@ def _torch():
     return torch
 
 
-def run_batch(entries, prec, xs, oracle64, check_idx=None, seed=0):
+def run_batch(entries, prec, xs, oracle64, check_idx=None, seed=0, mode=None):
     """entries: list of (kind, shape); xs: list of numpy inputs.  Packs the
     inputs into one device buffer with gaps, runs the batch, checks entries."""
     torch = _torch()
@@ -36,8 +36,9 @@ def run_batch(entries, prec, xs, oracle64, check_idx=None, seed=0):
         hin[o:o + x.nbytes] = np.frombuffer(np.ascontiguousarray(x).tobytes(), dtype=np.uint8)
     din = torch.from_numpy(hin).cuda()
     dout = torch.zeros(opos, dtype=torch.uint8, device="cuda")
-    b = mf.Batch([(k, s, io, oo) for (k, s), io, oo in zip(entries, ioffs, ooffs)],
-                 mf.PREC_FP64 if prec == 64 else mf.PREC_FP32)
+    if mode is None:
+        mode = mf.PREC_FP64 if prec == 64 else mf.PREC_FP32
+    b = mf.Batch([(k, s, io, oo) for (k, s), io, oo in zip(entries, ioffs, ooffs)], mode)
     st = torch.cuda.Stream()
     b.run(din.data_ptr(), dout.data_ptr(), st.cuda_stream)
     torch.cuda.synchronize()
@@ -144,3 +145,23 @@ def test_batch_rejects_bad_entries():
         mf.Batch([("dft", (16,), 4, 0)], mf.PREC_FP32)  # misaligned offset
     with pytest.raises(mf.MinfftError):
         mf.Batch([], mf.PREC_FP32)
+
+
+def test_split_fp16_mfma_batch(oracle64):
+    """MINFFT_SPLIT_FP16: 512-point axes take the split-FP16 MFMA radix-16
+    tile (contiguous and strided axes, forward and inverse); other lengths
+    the FP32 path.  Bar: rel-L2 <= 1e-6 vs the FP64 oracle."""
+    shapes = [(512,), (4, 512), (512, 4), (2, 512, 8), (512, 8, 2), (16, 512), (1024,), (512, 64)]
+    rng = np.random.default_rng(21)
+    entries, xs = [], []
+    for i in range(24):
+        shape = shapes[i % len(shapes)]
+        kind = "dft" if i % 2 == 0 else "invdft"
+        x = make_input(kind, shape, 32, int(rng.integers(1 << 30)))
+        if i % 3 == 1:
+            x = (x * np.float32(3e4)).astype(np.complex64)  # large values: per-column scaling
+        entries.append((kind, shape))
+        xs.append(x)
+    print("[case] split-FP16 batch", flush=True)
+    worst = run_batch(entries, 32, xs, oracle64, mode=mf.PREC_SPLIT_FP16)
+    print(f"split-FP16 worst rel-L2 {worst:.2e}")
