@@ -215,7 +215,7 @@ minfft_batch_plan* minfft_mkaux_batch(int count, const minfft_batch_entry* e, in
         size_t mx = 0;
         for (auto& g : p->groups)
             if (g.d == 1) mx = std::max(mx, g.ioff.size());
-        p->ws.nctrs = 8 + 2 * mx;
+        p->ws.nctrs = 8 + 2 * mx + 1;
         if (hipMalloc(&p->ws.ctrs, p->ws.nctrs * sizeof(unsigned)) != hipSuccess)
             return fail("hipMalloc batch counters failed");
     }

@@ -1,6 +1,11 @@
 // coop.hpp -- XCD-cooperative two-step kernel for batches of long 1D complex
 // transforms, keeping the intermediate in the XCD's L2.
 //
+// STATUS: disabled, unreachable from the engine.  Both GPU runs of it (FP64,
+// N = 2^14, 300 entries) hung -- the first with device-scope load polling,
+// the second with the RMW polling and bounded waits below -- and the cause
+// is not established.  Kept for study; do not enable without one.
+//
 // Same factorisation as big.hpp (N = N1*N2; step 1 = N2 columns of length N1
 // with the W_N^{n2*k1} twiddle, written transposed; step 2 = N1 rows of
 // length N2), but the work of one transform is spread over many work-groups
@@ -21,10 +26,18 @@
 // work-group, so the waits always end, however many work-groups are resident.
 //
 // Coherence.  All work-groups sharing a slot sit on the same XCD and share
-// its L2.  A producer waits for its stores to be acknowledged (s_waitcnt 0),
-// synchronises the work-group, then increments the chunk counter with an L2
-// atomic; a consumer polls the counter with device-scope loads and reads the
-// slot with device-scope (L1-bypassing) loads, so no stale L1 line is used.
+// its L2.  Counters are only ever touched by read-modify-write atomics
+// (claims, increments, and polls as "add 0"), which all execute at the same
+// place; device-scope atomic loads would bypass the cache and miss
+// increments that live in L2.  A producer waits for its scratch stores to be
+// acknowledged by L2 (s_waitcnt 0), synchronises the work-group and then
+// increments the chunk counter.  A consumer polls, synchronises, invalidates
+// its CU's vector L1 (buffer_inv sc0: a slot's old lines may still be cached
+// there from an earlier transform) and reads the slot with ordinary loads,
+// which hit the L2 lines the producers wrote.
+// Every wait is bounded: after kSpinLimit polls a work-group raises a shared
+// abort flag and all waits return at once, so a scheduling or coherence
+// failure ends the kernel (with wrong results) instead of hanging it.
 #pragma once
 #include "fft_core.hpp"
 
@@ -40,34 +53,35 @@ struct CoopArgs {
     const double2* tw64;
     const long long* in_off;  // entry offsets (points)
     const long long* out_off;
-    unsigned* ctrs;           // [kXcds heads][count done1][count done2], zeroed per launch
+    unsigned* ctrs;           // [kXcds heads][count done1][count done2][abort], zeroed per launch
     unsigned count;
     int slots;                // S
     int lookahead;            // LA < S
 };
 
-// device-scope (L2) load of one complex point, bypassing the CU's L1
-template <typename T> __device__ __forceinline__ c2<T> l2_load(const c2<T>* p);
-template <> __device__ __forceinline__ float2 l2_load<float>(const float2* p) {
-    unsigned long long v = __hip_atomic_load(reinterpret_cast<const unsigned long long*>(p), __ATOMIC_RELAXED,
-                                             __HIP_MEMORY_SCOPE_AGENT);
-    float2 r;
-    __builtin_memcpy(&r, &v, sizeof(r));
-    return r;
-}
-template <> __device__ __forceinline__ double2 l2_load<double>(const double2* p) {
-    const unsigned long long* q = reinterpret_cast<const unsigned long long*>(p);
-    unsigned long long a = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    unsigned long long b = __hip_atomic_load(q + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    double2 r;
-    __builtin_memcpy(&r.x, &a, sizeof(double));
-    __builtin_memcpy(&r.y, &b, sizeof(double));
-    return r;
+constexpr unsigned kSpinLimit = 1u << 22;
+
+// current value of a counter, read with an atomic RMW (add of a zero the
+// compiler cannot see) so that it executes where the increments do
+__device__ __forceinline__ unsigned rmw_read(unsigned* c) {
+    return __hip_atomic_fetch_add(c, (unsigned)opaque(0), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-__device__ __forceinline__ void wait_count(const unsigned* c, unsigned target) {
-    while (__hip_atomic_load(c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) __builtin_amdgcn_s_sleep(1);
+// wait until *c >= target; false if the shared abort flag is (or gets) set
+__device__ __forceinline__ bool wait_count(unsigned* c, unsigned target, unsigned* abort_flag) {
+    for (unsigned it = 0;; ++it) {
+        if (rmw_read(c) >= target) return true;
+        if (rmw_read(abort_flag) != 0) return false;
+        if (it >= kSpinLimit) {
+            __hip_atomic_fetch_add(abort_flag, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return false;
+        }
+        __builtin_amdgcn_s_sleep(1);
+    }
 }
+
+// invalidate this CU's vector L1 so later loads are served by L2
+__device__ __forceinline__ void l1_invalidate() { asm volatile("buffer_inv sc0" ::: "memory"); }
 
 static __global__ void coop_reset(unsigned* ctrs, unsigned n) {
     const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -102,6 +116,7 @@ __global__ void __launch_bounds__(BT, WPE) coop_kernel(CoopArgs A) {
     unsigned* head = A.ctrs + xcd;
     unsigned* done1 = A.ctrs + kXcds;
     unsigned* done2 = A.ctrs + kXcds + A.count;
+    unsigned* abort_flag = A.ctrs + kXcds + 2 * A.count;
     const unsigned G = (A.count > xcd) ? (A.count - xcd + kXcds - 1) / kXcds : 0;  // this XCD's transforms
     const int S = A.slots, LA = A.lookahead;
     const unsigned per_round = CH1 + CH2;
@@ -126,7 +141,7 @@ __global__ void __launch_bounds__(BT, WPE) coop_kernel(CoopArgs A) {
             const unsigned e = g * kXcds + xcd;
             C2* scr = reinterpret_cast<C2*>(A.scratch) + ((long long)xcd * S + (long long)(g % S)) * (long long)NN;
             if (g >= (unsigned)S) {
-                if (t == 0) wait_count(done2 + (e - (unsigned)S * kXcds), CH2);
+                if (t == 0) (void)wait_count(done2 + (e - (unsigned)S * kXcds), CH2, abort_flag);
                 __syncthreads();
             }
             const C2* __restrict__ x = in + A.in_off[e];
@@ -164,13 +179,14 @@ __global__ void __launch_bounds__(BT, WPE) coop_kernel(CoopArgs A) {
             const unsigned e = g * kXcds + xcd;
             const C2* scr = reinterpret_cast<const C2*>(A.scratch) +
                             ((long long)xcd * S + (long long)(g % S)) * (long long)NN;
-            if (t == 0) wait_count(done1 + e, CH1);
+            if (t == 0) (void)wait_count(done1 + e, CH1, abort_flag);
             __syncthreads();
+            l1_invalidate();
             C2* __restrict__ y = out + A.out_off[e];
             const int k1 = (int)(j - CH1) * CR + c2i;
             C2 v[E];
 #pragma unroll
-            for (int m = 0; m < E; ++m) v[m] = l2_load<T>(scr + (long long)(tl2 + m * TPT2) * N1 + k1);
+            for (int m = 0; m < E; ++m) v[m] = scr[(long long)(tl2 + m * TPT2) * N1 + k1];
             C2* slot = lds + c2i * SC2;
             Core2::run(v, slot, tl2, tw);
 #pragma unroll

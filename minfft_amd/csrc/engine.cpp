@@ -163,8 +163,8 @@ struct Tune {
     int fused = 1;      // fused two-step kernel for batches of long 1D entries
     int fused_wpc = 2;  // work-groups per CU of the fused kernel
     int fused_alt = 0;  // use the alternative (smaller) fused work-groups
-    int coop = 0;       // XCD-cooperative kernel (coop.hpp): off -- its first GPU
-                        // run hung with no cause shown yet; not used until one is
+    int coop = 0;       // XCD-cooperative kernel (coop.hpp): off and not selectable --
+                        // two GPU runs hung (see the header of coop.hpp)
     int coop_v = 0;     // its launch-shape variant
     int coop_wpc = 4;   // its work-groups per CU
     int coop_la = 3;    // queue lookahead (rounds)
@@ -194,7 +194,6 @@ static const Tune& tune() {
                 else if (k == "fused") g_tune.fused = v;
                 else if (k == "wpc" && v > 0) g_tune.fused_wpc = v;
                 else if (k == "falt") g_tune.fused_alt = v;
-                else if (k == "coop") g_tune.coop = v;
                 else if (k == "cv") g_tune.coop_v = v;
                 else if (k == "cwpc" && v > 0) g_tune.coop_wpc = v;
                 else if (k == "cla" && v >= 0) g_tune.coop_la = v;
@@ -427,7 +426,7 @@ static int c2c_axis(const Workspace& ws, Prec prec, int kind, int logL, const vo
         const unsigned cnt = lv.n ? lv.B[0] : 1;
         const int S = std::max(tn.coop_s, tn.coop_la + 1);
         const size_t need = (size_t)kXcds * S * ((size_t)1 << logL) * (prec == PREC_F64 ? 16 : 8);
-        const size_t nct = (size_t)kXcds + 2 * (size_t)cnt;
+        const size_t nct = (size_t)kXcds + 2 * (size_t)cnt + 1;
         if (fn && tn.coop && lv.n == 1 && lv.ioff && lv.ooff && e == 1 && cnt >= (unsigned)kXcds &&
             need <= ws.fs_bytes && ws.ctrs && nct <= ws.nctrs) {
             unsigned n = (unsigned)nct;
