@@ -79,9 +79,7 @@ __global__ void __launch_bounds__(512) mfma512_kernel(TileArgs A) {
         const int stl = rowm ? tl_row : tl_cf;
         const TransformRef src = (sc_ == c) ? me : locate(A, tile * (unsigned)C + (unsigned)sc_);
         C2* sslot = lds + (long long)sc_ * A.sc;
-        const C2* __restrict__ in = reinterpret_cast<const C2*>(A.in) + src.ib;
-        for (int i = stl; i < L; i += TPT)
-            sslot[lpad(i)] = src.valid ? in[(long long)i * A.in_e] : make_float2(0.f, 0.f);
+        if (src.valid) stage_in<float, INV ? K_INVDFT : K_DFT, L, TPT>(A, src, sslot, stl);
     }
     __syncthreads();
 
@@ -193,8 +191,7 @@ __global__ void __launch_bounds__(512) mfma512_kernel(TileArgs A) {
     const TransformRef dst = (sc_ == c) ? me : locate(A, tile * (unsigned)C + (unsigned)sc_);
     if (!dst.valid) return;
     const C2* sslot = lds + (long long)sc_ * A.sc;
-    C2* __restrict__ out = reinterpret_cast<C2*>(A.out) + dst.ob;
-    for (int i = stl; i < L; i += TPT) out[(long long)i * A.out_e] = sslot[lpad(i)];
+    stage_out<float, INV ? K_INVDFT : K_DFT, L, TPT>(A, dst, sslot, stl, tw);
 }
 
 }  // namespace mfx

@@ -86,6 +86,123 @@ __device__ __forceinline__ unsigned tile_id(const TileArgs& A) {
     return (g % X) * per + g / X;
 }
 
+// 16-byte vectors of reals, for contiguous real rows
+template <typename T> struct Vec16;
+template <> struct Vec16<float> { using V = float4; static constexpr int n = 4; };
+template <> struct Vec16<double> { using V = double2; static constexpr int n = 2; };
+
+// real item i of a slot that holds real items: the pair (2p, 2p+1) sits at
+// the padded complex position of p
+__device__ __forceinline__ int ridx(int i) { return 2 * lpad(i >> 1) + (i & 1); }
+
+// Staged input: this lane's share (items stl, stl + TPT, ...) of transform
+// `src` into its LDS slot.  Trip counts are compile-time constants, so the
+// loads of a lane are all issued before the first LDS store waits on them;
+// contiguous 16-byte-aligned real rows use 16-byte loads.
+template <typename T, int KIND, int L, int TPT>
+__device__ __forceinline__ void stage_in(const TileArgs& A, const TransformRef& src, c2<T>* sslot, int stl) {
+    using C2 = c2<T>;
+    if constexpr (kind_complex_in(KIND)) {
+        const C2* __restrict__ in = reinterpret_cast<const C2*>(A.in) + src.ib;
+        constexpr int K = L / TPT;
+        const long long es = A.in_e;
+        C2 r[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) r[k] = in[(long long)(stl + k * TPT) * es];
+#pragma unroll
+        for (int k = 0; k < K; ++k) sslot[lpad(stl + k * TPT)] = r[k];
+        if constexpr (kind_nin(KIND, L) > L) {  // inverse real DFT: item L
+            if (stl == 0) sslot[lpad(L)] = in[(long long)L * es];
+        }
+    } else {
+        // 2L real items (the real DFT's input pairs, or a DCT/DST row)
+        constexpr int NR = 2 * L;
+        using V = typename Vec16<T>::V;
+        constexpr int VN = Vec16<T>::n;
+        const T* __restrict__ in = reinterpret_cast<const T*>(A.in) + src.ib;
+        T* rs = reinterpret_cast<T*>(sslot);
+        if constexpr (NR % (VN * TPT) == 0) {
+            if (A.in_e == 1 && (reinterpret_cast<unsigned long long>(in) & 15) == 0) {
+                constexpr int K = NR / (VN * TPT);
+                V r[K];
+#pragma unroll
+                for (int k = 0; k < K; ++k) r[k] = reinterpret_cast<const V*>(in)[stl + k * TPT];
+#pragma unroll
+                for (int k = 0; k < K; ++k) {
+                    const int p = (VN / 2) * (stl + k * TPT);  // first complex position
+                    if constexpr (VN == 4) {
+                        // positions p, p+1 (p even: never split by a pad gap)
+                        C2* d = sslot + lpad(p);
+                        d[0] = mk<T>(r[k].x, r[k].y);
+                        d[1] = mk<T>(r[k].z, r[k].w);
+                    } else {
+                        sslot[lpad(p)] = mk<T>(r[k].x, r[k].y);
+                    }
+                }
+                return;
+            }
+        }
+        constexpr int K = NR / TPT;
+        const long long es = A.in_e;
+        T r[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) r[k] = in[(long long)(stl + k * TPT) * es];
+#pragma unroll
+        for (int k = 0; k < K; ++k) rs[ridx(stl + k * TPT)] = r[k];
+    }
+}
+
+// Staged output: the kind's output items of transform `dst` from the FFT
+// output in its LDS slot (post-processing fused), this lane's share.
+template <typename T, int KIND, int L, int TPT>
+__device__ __forceinline__ void stage_out(const TileArgs& A, const TransformRef& dst, const c2<T>* sslot, int stl,
+                                          const c2<T>* __restrict__ tw) {
+    using C2 = c2<T>;
+    auto Tf = [&](int p) -> C2 { return sslot[lpad(p)]; };
+    if constexpr (kind_complex_out(KIND)) {
+        C2* __restrict__ out = reinterpret_cast<C2*>(A.out) + dst.ob;
+        constexpr int K = L / TPT;
+        const long long es = A.out_e;
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const int i = stl + k * TPT;
+            out[(long long)i * es] = post_item<T, KIND>(i, L, tw, Tf);
+        }
+        if constexpr (kind_nout(KIND, L) > L) {  // real DFT: item L
+            if (stl == 0) out[(long long)L * es] = post_item<T, KIND>(L, L, tw, Tf);
+        }
+    } else {
+        constexpr int NR = 2 * L;
+        using V = typename Vec16<T>::V;
+        constexpr int VN = Vec16<T>::n;
+        T* __restrict__ out = reinterpret_cast<T*>(A.out) + dst.ob;
+        if constexpr (NR % (VN * TPT) == 0) {
+            if (A.out_e == 1 && (reinterpret_cast<unsigned long long>(out) & 15) == 0) {
+                constexpr int K = NR / (VN * TPT);
+#pragma unroll
+                for (int k = 0; k < K; ++k) {
+                    const int j = stl + k * TPT;
+                    T w[VN];
+#pragma unroll
+                    for (int q = 0; q < VN; ++q) w[q] = post_item<T, KIND>(VN * j + q, L, tw, Tf).x;
+                    V o;
+                    if constexpr (VN == 4) o = make_float4(w[0], w[1], w[2], w[3]);
+                    else o = make_double2(w[0], w[1]);
+                    reinterpret_cast<V*>(out)[j] = o;
+                }
+                return;
+            }
+        }
+        constexpr int K = NR / TPT;
+        const long long es = A.out_e;
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const int i = stl + k * TPT;
+            out[(long long)i * es] = post_item<T, KIND>(i, L, tw, Tf).x;
+        }
+    }
+}
+
 // Largest work-group of a tile kernel: FP64 radix-16 butterflies need more
 // than the 128 VGPRs a 1024-thread group leaves each lane.
 template <typename T> constexpr int max_threads() { return sizeof(T) == 8 ? 512 : 1024; }
@@ -142,27 +259,10 @@ __global__ void __launch_bounds__(max_threads<T>()) tile_kernel(TileArgs A) {
         const int stl = rowm ? tl_row : tl_cf;
         const TransformRef src = (sc_ == c) ? me : locate(A, tile * (unsigned)C + (unsigned)sc_);
         C2* sslot = lds + (long long)sc_ * A.sc;
-        if (src.valid) {
-            if constexpr (CIN || KIND == K_REALDFT) {
-                // complex items (for the real DFT: pairs x[2p], x[2p+1])
-                const long long es = (KIND == K_REALDFT) ? 1 : A.in_e;
-                const long long base = (KIND == K_REALDFT) ? (src.ib >> 1) : src.ib;
-                const C2* __restrict__ in = reinterpret_cast<const C2*>(A.in) + base;
-                if (es == 1) {
-                    for (int i = stl; i < NIN; i += TPT) sslot[lpad(i)] = in[i];
-                } else {
-                    for (int i = stl; i < NIN; i += TPT) sslot[lpad(i)] = in[(long long)i * es];
-                }
-            } else {
-                const T* __restrict__ in = reinterpret_cast<const T*>(A.in);
-                T* rs = reinterpret_cast<T*>(sslot);
-                for (int i = stl; i < NIN; i += TPT)
-                    rs[2 * lpad(i >> 1) + (i & 1)] = in[src.ib + (long long)i * A.in_e];
-            }
-        }
+        if (src.valid) stage_in<T, KIND, L, TPT>(A, src, sslot, stl);
         __syncthreads();
         const T* rslot = reinterpret_cast<const T*>(slot);
-        auto X = [&](int i) -> T { return rslot[2 * lpad(i >> 1) + (i & 1)]; };
+        auto X = [&](int i) -> T { return rslot[ridx(i)]; };
         auto Z = [&](int i) -> C2 { return slot[lpad(i)]; };
         if constexpr (KIND == K_DFT || KIND == K_INVDFT || KIND == K_REALDFT) {
             Core<T, LOGL, E, INV>::read_owned(v, slot, tl);
@@ -224,16 +324,7 @@ __global__ void __launch_bounds__(max_threads<T>()) tile_kernel(TileArgs A) {
         const TransformRef dst = (sc_ == c) ? me : locate(A, tile * (unsigned)C + (unsigned)sc_);
         const C2* sslot = lds + (long long)sc_ * A.sc;
         if (!dst.valid) return;
-        auto Tf = [&](int p) -> C2 { return sslot[lpad(p)]; };
-        if constexpr (COUT) {
-            C2* __restrict__ out = reinterpret_cast<C2*>(A.out);
-            for (int i = stl; i < NOUT; i += TPT)
-                out[dst.ob + (long long)i * A.out_e] = post_item<T, KIND>(i, L, tw, Tf);
-        } else {
-            T* __restrict__ out = reinterpret_cast<T*>(A.out);
-            for (int i = stl; i < NOUT; i += TPT)
-                out[dst.ob + (long long)i * A.out_e] = post_item<T, KIND>(i, L, tw, Tf).x;
-        }
+        stage_out<T, KIND, L, TPT>(A, dst, sslot, stl, tw);
     }
 }
 

@@ -123,6 +123,53 @@ def test_mixed_real_kinds_2d_batch(oracle64, prec):
     run_batch(entries, prec, xs, oracle64)
 
 
+@pytest.mark.parametrize("prec", [32, 64])
+def test_real_kinds_odd_offsets(oracle64, prec):
+    """Real rows that start at an odd real index (not 16-byte aligned): the
+    staged loads and stores take the scalar path, and the real DFT's input
+    pairs straddle complex boundaries."""
+    torch = _torch()
+    rdt = np.float32 if prec == 32 else np.float64
+    cdt = np.complex64 if prec == 32 else np.complex128
+    rs = np.dtype(rdt).itemsize
+    kinds = ["realdft", "dct2", "dst3", "dct4", "dst2", "realdft"]
+    shapes = [(64,), (8, 32), (256,), (4, 64), (1024,), (2, 16)]
+    entries, xs, ioffs, ooffs = [], [], [], []
+    ipos, opos = 3 * rs, 0
+    for i, (kind, shape) in enumerate(zip(kinds, shapes)):
+        x = make_input(kind, shape, prec, 500 + i)
+        entries.append((kind, shape))
+        xs.append(x)
+        ioffs.append(ipos)
+        ipos += x.nbytes + 5 * rs  # odd real count between entries
+        n_out = int(np.prod(out_shape(kind, shape))) * (2 if is_complex_out(kind) else 1)
+        if is_complex_out(kind):
+            opos = -(-opos // (2 * rs)) * (2 * rs)  # complex items: complex-aligned
+        elif (opos // rs) % 2 == 0:
+            opos += rs  # real rows: start at an odd real index
+        ooffs.append(opos)
+        opos += n_out * rs + rs
+    hin = np.zeros(ipos, dtype=np.uint8)
+    for x, o in zip(xs, ioffs):
+        hin[o:o + x.nbytes] = np.frombuffer(np.ascontiguousarray(x).tobytes(), dtype=np.uint8)
+    din = torch.from_numpy(hin).cuda()
+    dout = torch.zeros(opos, dtype=torch.uint8, device="cuda")
+    b = mf.Batch([(k, s, io, oo) for (k, s), io, oo in zip(entries, ioffs, ooffs)],
+                 mf.PREC_FP64 if prec == 64 else mf.PREC_FP32)
+    print(f"[case] odd offsets prec={prec}", flush=True)
+    b.run(din.data_ptr(), dout.data_ptr(), None)
+    torch.cuda.synchronize()
+    hout = dout.cpu().numpy()
+    b.close()
+    for (kind, shape), x, oo in zip(entries, xs, ooffs):
+        odt = cdt if is_complex_out(kind) else rdt
+        n_out = int(np.prod(out_shape(kind, shape)))
+        y = hout[oo:oo + n_out * np.dtype(odt).itemsize].view(odt).reshape(out_shape(kind, shape))
+        ref = oracle64.run(kind, x.astype(np.float64), shape)
+        bar = tol64(shape) if prec == 64 else TOL32
+        assert rel_l2(y, ref) <= bar, (kind, shape, prec)
+
+
 def test_batch_host_pointers(oracle64):
     """minfft_batch: host buffers, synchronous."""
     sizes = [64, 1024, 4096, 256]
