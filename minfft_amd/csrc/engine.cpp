@@ -306,8 +306,10 @@ static int launch_tile(Prec prec, int kind, int logL, const void* tw, const void
     const int maxT = prec == PREC_F64 ? 512 : 1024;  // max_threads<T>() of tile.hpp
     int C;
     if (wants_cf) {
-        // several columns per tile so each row segment spans >= 64 B
-        C = std::max(1, (int)(tn.cfb / cs));
+        // several columns per tile so that each row segment of the strided
+        // side spans cfb bytes (real items for the DCT/DST kinds)
+        const size_t isz = cplx || kind == K_REALDFT || kind == K_INVREALDFT ? cs : cs / 2;
+        C = std::max(1, (int)(tn.cfb / isz));
         while (C > 1 && C * TPT > maxT) C >>= 1;
     } else {
         C = std::max(1, std::min(tn.tpw, maxT) / TPT);

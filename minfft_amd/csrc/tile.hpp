@@ -203,6 +203,196 @@ __device__ __forceinline__ void stage_out(const TileArgs& A, const TransformRef&
     }
 }
 
+// ---------------------------------------------------------------------------
+// Pre- and post-processing of the real kinds inside the tile kernel, in
+// symmetric groups so that the unrolled loops carry no per-item special
+// cases: the only branches are for the lane holding point 0 (and L/2).  The
+// formulas are those of kinds.hpp (same references), regrouped:
+//   * the C2R pre-twiddle (inverse real DFT, DCT-3, DST-3) of points p and
+//     L-p shares z[m], z[L-m] and W_N^m, m = min(p, L-p);
+//   * the R2C post-twiddle (real DFT, DCT-2, DST-2) of group m gives Z[m]
+//     and Z[L-m] from T[m], T[L-m] and W_N^m; DCT-2/DST-2 turn each Z[n]
+//     into two outputs (real and imaginary part of Z[n]*W_{4N}^n);
+//   * DCT-4/DST-4 and the un-reorders produce output pairs (2j, 2j+1).
+// Used for E >= 2 (L >= 2); length-1 FFTs use the per-item formulas.
+// ---------------------------------------------------------------------------
+
+// FFT input points tl + m*TPT of this lane, from the kind's input staged in
+// its own LDS slot (real items at ridx, complex items at lpad)
+template <typename T, int KIND, int L, int E, int TPT>
+__device__ __forceinline__ void pre_owned(c2<T>* v, const c2<T>* slot, int tl, const c2<T>* __restrict__ tw) {
+    using C2 = c2<T>;
+    static_assert(E >= 2 && E % 2 == 0, "pre_owned needs E >= 2");
+    constexpr int N = 2 * L, H = L / 2;
+    const T* rs = reinterpret_cast<const T*>(slot);
+    auto X = [&](int i) -> T { return rs[ridx(i)]; };
+    if constexpr (KIND == K_DCT2 || KIND == K_DST2) {
+        // t[n] = x[2n], t[L+n] = s*x[N-1-2n]; point p = (t[2p], t[2p+1])
+        constexpr T sg = (KIND == K_DST2) ? T(-1) : T(1);
+#pragma unroll
+        for (int m = 0; m < E; ++m) {
+            const int p = tl + m * TPT;
+            if (m < E / 2) {
+                v[m] = mk<T>(X(4 * p), X(4 * p + 2));
+            } else {
+                const int q = p - H;
+                v[m] = mk<T>(sg * X(N - 1 - 4 * q), sg * X(N - 3 - 4 * q));
+            }
+        }
+    } else if constexpr (KIND == K_DCT4 || KIND == K_DST4) {
+#pragma unroll
+        for (int m = 0; m < E; ++m) {
+            const int p = tl + m * TPT;
+            const C2 e = tw[2 * N + p];
+            const T xa = X(2 * p), xb = X(N - 1 - 2 * p);
+            if constexpr (KIND == K_DCT4)
+                v[m] = mk<T>(e.x * xa - e.y * xb, e.x * xb + e.y * xa);
+            else
+                v[m] = mk<T>(-e.x * xa - e.y * xb, e.x * xb - e.y * xa);
+        }
+    } else {  // inverse real DFT, DCT-3, DST-3: C2R pre-twiddle of z
+        constexpr bool T3 = (KIND == K_DCT3 || KIND == K_DST3);
+        constexpr bool DS = (KIND == K_DST3);
+        // z[k] for 0 < k < L
+        auto zg = [&](int k) -> C2 {
+            if constexpr (T3) {
+                const C2 e = tw[4 * N + k];
+                const T xa = DS ? X(N - k - 1) : X(k);
+                const T xb = DS ? X(k - 1) : X(N - k);
+                return mk<T>(xa * e.x - xb * e.y, -xa * e.y - xb * e.x);
+            } else {
+                return slot[lpad(k)];
+            }
+        };
+#pragma unroll
+        for (int m = 0; m < E; ++m) {
+            const int p = tl + m * TPT;
+            const bool lower = m < E / 2;  // p < L/2
+            const int mm = lower ? p : L - p;
+            if ((m == 0 || m == E / 2) && tl == 0) {
+                if (m == 0) {
+                    // point 0 from z[0], z[L] (imaginary parts ignored)
+                    T a, b;
+                    if constexpr (T3) {
+                        a = DS ? X(N - 1) : X(0);
+                        b = sqrt2_v<T>() * (DS ? X(L - 1) : X(L));
+                    } else {
+                        a = slot[lpad(0)].x;
+                        b = slot[lpad(L)].x;
+                    }
+                    v[m] = mk<T>(a + b, a - b);
+                } else {
+                    const C2 z = zg(H);
+                    v[m] = mk<T>(2 * z.x, -2 * z.y);
+                }
+                continue;
+            }
+            const C2 a = zg(mm), b = zg(L - mm);
+            const C2 e = tw[2 * L + mm];
+            const T ur = a.x + b.x, ui = a.y - b.y;
+            const T ttr = a.x - b.x, tti = a.y + b.y;
+            const T vr = ttr * e.y - tti * e.x;  // (ttr + i tti) * i conj(e)
+            const T vi = ttr * e.x + tti * e.y;
+            v[m] = lower ? mk<T>(ur + vr, ui + vi) : mk<T>(ur - vr, -ui + vi);
+        }
+    }
+}
+
+// two real outputs (2j, 2j+1) of a row; one 2-wide store when contiguous
+template <typename T>
+__device__ __forceinline__ void store_pair(T* __restrict__ out, long long es, bool vec, int j, T a, T b) {
+    if (vec) {
+        reinterpret_cast<c2<T>*>(out)[j] = mk<T>(a, b);
+    } else {
+        out[(long long)(2 * j) * es] = a;
+        out[(long long)(2 * j + 1) * es] = b;
+    }
+}
+
+// the kind's outputs of transform dst from the FFT output in its LDS slot
+template <typename T, int KIND, int L, int E, int TPT>
+__device__ __forceinline__ void post_real(const TileArgs& A, const TransformRef& dst, const c2<T>* sslot, int stl,
+                                          const c2<T>* __restrict__ tw) {
+    using C2 = c2<T>;
+    static_assert(E >= 2 && E % 2 == 0, "post_real needs E >= 2");
+    constexpr int N = 2 * L, H = L / 2;
+    auto Tf = [&](int p) -> C2 { return sslot[lpad(p)]; };
+    const long long es = A.out_e;
+    if constexpr (KIND == K_REALDFT || KIND == K_DCT2 || KIND == K_DST2) {
+        // Z[n] -> outputs.  Real DFT: out[n] = Z[n].  DCT-2 / DST-2: with
+        // c = W_{4N}^n, Re(Z c) goes to item re(n), -Im(Z c) to item im(n).
+        constexpr bool DS = (KIND == K_DST2);
+        auto re_pos = [&](int n) -> int { return DS ? N - 1 - n : n; };
+        auto im_pos = [&](int n) -> int { return DS ? n - 1 : N - n; };
+        C2* __restrict__ oc = reinterpret_cast<C2*>(A.out) + dst.ob;
+        T* __restrict__ orl = reinterpret_cast<T*>(A.out) + dst.ob;
+        auto emit = [&](int n, C2 z) {
+            if constexpr (KIND == K_REALDFT) {
+                oc[(long long)n * es] = z;
+            } else {
+                const C2 c = tw[4 * N + n];
+                orl[(long long)re_pos(n) * es] = 2 * (z.x * c.x - z.y * c.y);
+                orl[(long long)im_pos(n) * es] = -2 * (z.x * c.y + z.y * c.x);
+            }
+        };
+#pragma unroll
+        for (int k = 0; k < E / 2; ++k) {
+            const int m = stl + k * TPT;  // group (m, L-m), 0 <= m < L/2
+            if (k == 0 && stl == 0) {
+                const C2 u = Tf(0);
+                const C2 zh = conj<T>(Tf(H));
+                if constexpr (KIND == K_REALDFT) {
+                    oc[0] = mk<T>(u.x + u.y, (T)0);
+                    oc[(long long)L * es] = mk<T>(u.x - u.y, (T)0);
+                } else {
+                    orl[(long long)re_pos(0) * es] = 2 * (u.x + u.y);
+                    orl[(long long)re_pos(L) * es] = sqrt2_v<T>() * (u.x - u.y);
+                }
+                emit(H, zh);
+                continue;
+            }
+            const C2 a = Tf(m), b = Tf(L - m);
+            const C2 e = tw[2 * L + m];
+            const T ur = (a.x + b.x) / 2, ui = (a.y - b.y) / 2;
+            const T ttr = a.x - b.x, tti = a.y + b.y;
+            const T vr = (ttr * e.y + tti * e.x) / 2;  // (ttr + i tti) * e / i / 2
+            const T vi = (-ttr * e.x + tti * e.y) / 2;
+            emit(m, mk<T>(ur + vr, ui + vi));
+            emit(L - m, mk<T>(ur - vr, -ui + vi));
+        }
+    } else {
+        T* __restrict__ out = reinterpret_cast<T*>(A.out) + dst.ob;
+        const bool vec = es == 1 && (reinterpret_cast<unsigned long long>(out) & (2 * sizeof(T) - 1)) == 0;
+#pragma unroll
+        for (int k = 0; k < E; ++k) {
+            const int j = stl + k * TPT;  // 0 <= j < L
+            if constexpr (KIND == K_INVREALDFT) {
+                const C2 t = Tf(j);
+                store_pair<T>(out, es, vec, j, t.x, t.y);
+            } else if constexpr (KIND == K_DCT3 || KIND == K_DST3) {
+                // y[2j] = u[j], y[2j+1] = s*u[N-1-j], u = FFT output as reals
+                constexpr T sg = (KIND == K_DST3) ? T(-1) : T(1);
+                const C2 t0 = Tf(j >> 1);
+                const int r = N - 1 - j;
+                const C2 t1 = Tf(r >> 1);
+                store_pair<T>(out, es, vec, j, (j & 1) ? t0.y : t0.x, sg * ((r & 1) ? t1.y : t1.x));
+            } else {  // DCT-4 / DST-4
+                const C2 e0 = tw[8 * N + 4 * j + 1], e1 = tw[8 * N + 4 * j + 3];
+                const C2 t0 = Tf(j), t1 = Tf(L - 1 - j);
+                T y0, y1;
+                if constexpr (KIND == K_DCT4) {
+                    y0 = 2 * (e0.x * t0.x - e0.y * t0.y);
+                    y1 = 2 * (e1.x * t1.x + e1.y * t1.y);
+                } else {
+                    y0 = 2 * (e0.x * t0.y + e0.y * t0.x);
+                    y1 = 2 * (-e1.x * t1.y + e1.y * t1.x);
+                }
+                store_pair<T>(out, es, vec, j, y0, y1);
+            }
+        }
+    }
+}
+
 // Largest work-group of a tile kernel: FP64 radix-16 butterflies need more
 // than the 128 VGPRs a 1024-thread group leaves each lane.
 template <typename T> constexpr int max_threads() { return sizeof(T) == 8 ? 512 : 1024; }
@@ -266,6 +456,8 @@ __global__ void __launch_bounds__(max_threads<T>()) tile_kernel(TileArgs A) {
         auto Z = [&](int i) -> C2 { return slot[lpad(i)]; };
         if constexpr (KIND == K_DFT || KIND == K_INVDFT || KIND == K_REALDFT) {
             Core<T, LOGL, E, INV>::read_owned(v, slot, tl);
+        } else if constexpr (E >= 2) {
+            pre_owned<T, KIND, L, E, TPT>(v, slot, tl, tw);
         } else {
 #pragma unroll
             for (int m = 0; m < E; ++m) {
@@ -324,7 +516,10 @@ __global__ void __launch_bounds__(max_threads<T>()) tile_kernel(TileArgs A) {
         const TransformRef dst = (sc_ == c) ? me : locate(A, tile * (unsigned)C + (unsigned)sc_);
         const C2* sslot = lds + (long long)sc_ * A.sc;
         if (!dst.valid) return;
-        stage_out<T, KIND, L, TPT>(A, dst, sslot, stl, tw);
+        if constexpr (KIND != K_DFT && KIND != K_INVDFT && E >= 2)
+            post_real<T, KIND, L, E, TPT>(A, dst, sslot, stl, tw);
+        else
+            stage_out<T, KIND, L, TPT>(A, dst, sslot, stl, tw);
     }
 }
 
