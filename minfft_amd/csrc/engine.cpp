@@ -157,6 +157,7 @@ bool is_device_pointer(const void* p) {
 struct Tune {
     int tpw = 256;
     int cfb = 128;  // bytes per row segment of a c-fast tile
+    int wave = 1;   // wavefront-local exchanges for row tiles of <= 64 lanes per transform
     int xcd = 1;
     int stage = 0;
     int chunk_mb = 64;  // two-step chunk size for entry batches
@@ -188,6 +189,7 @@ static const Tune& tune() {
                 int v = std::atoi(kv.c_str() + eq + 1);
                 if (k == "tpw" && v > 0) g_tune.tpw = v;
                 else if (k == "cfb" && v > 0) g_tune.cfb = v;
+                else if (k == "wave") g_tune.wave = v;
                 else if (k == "xcd") g_tune.xcd = v;
                 else if (k == "stage") g_tune.stage = v;
                 else if (k == "chunk" && v > 0) g_tune.chunk_mb = v;
@@ -327,6 +329,7 @@ static int launch_tile(Prec prec, int kind, int logL, const void* tw, const void
     const unsigned long long tiles = (ntr + C - 1) / C;
     A.ntiles = (unsigned)tiles;
     A.xcd = (tn.xcd && wants_cf && tiles >= 2ull * D.nxcd) ? D.nxcd : 1;
+    A.wave = (tn.wave && !wants_cf) ? 1 : 0;
     void* args[] = {&A};
     return launch_fn(fn, (unsigned)tiles, (unsigned)(C * TPT), args, (size_t)C * sc * cs, st);
 }

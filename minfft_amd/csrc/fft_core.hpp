@@ -153,6 +153,31 @@ constexpr int lpad_c(int p) { return p + (p >> 4); }
 // neighbouring slots start in different banks.
 constexpr int slot_len(int L) { return (lpad_c(L) + 1) | 1; }
 
+// Twiddle-table read at an unsigned index: the byte offset is formed in 32
+// bits from the (uniform) table base, which lets the compiler use the
+// scalar-base + 32-bit vector-offset load form (one address register per
+// lane instead of a 64-bit pair).  Tables stay far below 4 GiB.
+template <typename T>
+__device__ __forceinline__ c2<T> twl(const c2<T>* __restrict__ tw, unsigned i) {
+    return *reinterpret_cast<const c2<T>*>(reinterpret_cast<const char*>(tw) + i * (unsigned)sizeof(c2<T>));
+}
+
+// Barrier between the LDS write and the LDS read of an exchange.  When every
+// transform of the work-group lies inside one wavefront (wave == true), the
+// exchange only needs ordering inside the wavefront: a wavefront's LDS
+// operations execute in issue order, so a compiler-level fence suffices and
+// the wavefronts of the work-group run independently.  Otherwise a
+// work-group barrier.  `wave` must be uniform over the work-group.
+__device__ __forceinline__ void tile_sync(bool wave) {
+    if (wave) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    } else {
+        __syncthreads();
+    }
+}
+
 template <typename T, int LOGL, int E, bool INV>
 struct Core {
     static constexpr int L = 1 << LOGL;
@@ -181,7 +206,7 @@ struct Core {
                 for (int b = 0; b < NB; ++b) {
                     const int k = (tlo + b * TPT) & (Ns - 1);
 #pragma unroll
-                    for (int r = 1; r < R; ++r) w[b * (R - 1) + r - 1] = twr[r * k];
+                    for (int r = 1; r < R; ++r) w[b * (R - 1) + r - 1] = twl<T>(twr, (unsigned)(r * k));
                 }
             }
         }
@@ -211,16 +236,18 @@ struct Core {
 
     // slot: this transform's LDS slot (complex points, padded by lpad).
     // On entry the slot may be overwritten (caller has synchronised).
-    __device__ static void run(c2<T> (&v)[E], c2<T>* slot, int tl, const c2<T>* __restrict__ tw) {
+    // wave: every transform of the work-group lies in one wavefront (TPT <= 64)
+    __device__ static void run(c2<T> (&v)[E], c2<T>* slot, int tl, const c2<T>* __restrict__ tw,
+                               bool wave = false) {
         if constexpr (NP > 0) {
             Tw<0> w0;
-            pass<0>(v, slot, tl, tw, w0);
+            pass<0>(v, slot, tl, tw, w0, wave);
         }
     }
 
     template <int P>
     __device__ static void pass(c2<T> (&v)[E], c2<T>* slot, int tl, const c2<T>* __restrict__ tw,
-                                const Tw<P>& twp) {
+                                const Tw<P>& twp, bool wave) {
         constexpr int R = Tw<P>::R;
         constexpr int Ns = Tw<P>::Ns;
         constexpr int NB = Tw<P>::NB;
@@ -260,10 +287,10 @@ struct Core {
         if constexpr (!last) {
             Tw<P + 1> nxt;
             nxt.load(tl, tw);
-            __syncthreads();
+            tile_sync(wave);
             read_owned(v, slot, tl);
-            __syncthreads();
-            pass<P + 1>(v, slot, tl, tw, nxt);
+            tile_sync(wave);
+            pass<P + 1>(v, slot, tl, tw, nxt, wave);
         }
     }
 };
@@ -287,7 +314,7 @@ __device__ __forceinline__ void stockham_pass(c2<T> (&v)[E], c2<T>* slot, int tl
             const c2<T>* twr = tw + Ns * R;
 #pragma unroll
             for (int r = 1; r < R; ++r) {
-                const c2<T> w = twr[r * k];
+                const c2<T> w = twl<T>(twr, (unsigned)(r * k));
                 a[r] = INV ? cmulc<T>(a[r], w) : cmul<T>(a[r], w);
             }
         }

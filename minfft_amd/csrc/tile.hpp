@@ -42,6 +42,8 @@ struct TileArgs {
     int fs_logN;                       // 0: none; else multiply by W_{2^fs_logN}^{n2*k}
     int sc;                            // LDS slot length (complex points)
     int xcd;                           // number of XCDs for the tile swizzle (1 = off)
+    int wave;                          // 1: row mappings only, so a transform's
+                                       // lanes share one wavefront when TPT <= 64
 };
 
 struct TransformRef {
@@ -243,7 +245,7 @@ __device__ __forceinline__ void pre_owned(c2<T>* v, const c2<T>* slot, int tl, c
 #pragma unroll
         for (int m = 0; m < E; ++m) {
             const int p = tl + m * TPT;
-            const C2 e = tw[2 * N + p];
+            const C2 e = twl<T>(tw, (unsigned)(2 * N + p));
             const T xa = X(2 * p), xb = X(N - 1 - 2 * p);
             if constexpr (KIND == K_DCT4)
                 v[m] = mk<T>(e.x * xa - e.y * xb, e.x * xb + e.y * xa);
@@ -251,51 +253,83 @@ __device__ __forceinline__ void pre_owned(c2<T>* v, const c2<T>* slot, int tl, c
                 v[m] = mk<T>(-e.x * xa - e.y * xb, e.x * xb - e.y * xa);
         }
     } else {  // inverse real DFT, DCT-3, DST-3: C2R pre-twiddle of z
-        constexpr bool T3 = (KIND == K_DCT3 || KIND == K_DST3);
-        constexpr bool DS = (KIND == K_DST3);
-        // z[k] for 0 < k < L
-        auto zg = [&](int k) -> C2 {
-            if constexpr (T3) {
-                const C2 e = tw[4 * N + k];
-                const T xa = DS ? X(N - k - 1) : X(k);
-                const T xb = DS ? X(k - 1) : X(N - k);
-                return mk<T>(xa * e.x - xb * e.y, -xa * e.y - xb * e.x);
-            } else {
-                return slot[lpad(k)];
-            }
-        };
+        // z[0..L] sits in the slot (the inverse real DFT's input; for
+        // DCT-3/DST-3 put there by t3_to_z)
+        auto zg = [&](int k) -> C2 { return slot[lpad(k)]; };
 #pragma unroll
         for (int m = 0; m < E; ++m) {
             const int p = tl + m * TPT;
             const bool lower = m < E / 2;  // p < L/2
-            const int mm = lower ? p : L - p;
-            if ((m == 0 || m == E / 2) && tl == 0) {
-                if (m == 0) {
-                    // point 0 from z[0], z[L] (imaginary parts ignored)
-                    T a, b;
-                    if constexpr (T3) {
-                        a = DS ? X(N - 1) : X(0);
-                        b = sqrt2_v<T>() * (DS ? X(L - 1) : X(L));
-                    } else {
-                        a = slot[lpad(0)].x;
-                        b = slot[lpad(L)].x;
-                    }
-                    v[m] = mk<T>(a + b, a - b);
-                } else {
-                    const C2 z = zg(H);
-                    v[m] = mk<T>(2 * z.x, -2 * z.y);
-                }
-                continue;
-            }
+            // in [1, L/2] (point 0, whose result is replaced below, is
+            // clamped to 1 so that every read stays inside the slot)
+            const int mm = max(lower ? p : L - p, 1);
             const C2 a = zg(mm), b = zg(L - mm);
-            const C2 e = tw[2 * L + mm];
+            const C2 e = twl<T>(tw, (unsigned)(2 * L + mm));
             const T ur = a.x + b.x, ui = a.y - b.y;
             const T ttr = a.x - b.x, tti = a.y + b.y;
             const T vr = ttr * e.y - tti * e.x;  // (ttr + i tti) * i conj(e)
             const T vi = ttr * e.x + tti * e.y;
-            v[m] = lower ? mk<T>(ur + vr, ui + vi) : mk<T>(ur - vr, -ui + vi);
+            C2 r = lower ? mk<T>(ur + vr, ui + vi) : mk<T>(ur - vr, -ui + vi);
+            if (m == 0) {
+                // point 0 from z[0], z[L] (imaginary parts ignored), selected
+                // by the lane that holds it
+                const T a0 = zg(0).x, b0 = zg(L).x;
+                r = (tl == 0) ? mk<T>(a0 + b0, a0 - b0) : r;
+            }
+            if (m == E / 2) {
+                // point L/2: 2 conj(z[L/2])
+                const C2 zh = zg(H);
+                r = (tl == 0) ? mk<T>(2 * zh.x, -2 * zh.y) : r;
+            }
+            v[m] = r;
         }
     }
+}
+
+// DCT-3 / DST-3: the staged real input x (2L items) -> z[0..L] in the same
+// slot (kinds.hpp t3_z, minfft.c:807-817, 857-867).  Group g < L/2 of lane tl
+// (g = tl + k*TPT) reads x at g, N-g, L-g, L+g (or their DST-3 mirrors) and
+// yields z[g], z[L-g]; the lane of g = 0 yields z[0], z[L], z[L/2].  All
+// reads happen before the barrier, all writes after it.
+template <typename T, int KIND, int L, int E, int TPT>
+__device__ __forceinline__ void t3_to_z(c2<T>* slot, int tl, const c2<T>* __restrict__ tw, bool wave) {
+    using C2 = c2<T>;
+    static_assert(E >= 2 && E % 2 == 0, "t3_to_z needs E >= 2");
+    constexpr int N = 2 * L, H = L / 2;
+    constexpr bool DS = (KIND == K_DST3);
+    const T* rs = reinterpret_cast<const T*>(slot);
+    auto X = [&](int i) -> T { return rs[ridx(i)]; };
+    // z[k], 0 < k < L
+    auto zg = [&](int k) -> C2 {
+        const C2 e = twl<T>(tw, (unsigned)(4 * N + k));
+        const T xa = DS ? X(N - k - 1) : X(k);
+        const T xb = DS ? X(k - 1) : X(N - k);
+        return mk<T>(xa * e.x - xb * e.y, -xa * e.y - xb * e.x);
+    };
+    constexpr int G = E / 2;
+    C2 za[G], zb[G];
+#pragma unroll
+    for (int k = 0; k < G; ++k) {
+        const int g = max(tl + k * TPT, 1);  // g = 0 is replaced below
+        za[k] = zg(g);
+        zb[k] = zg(L - g);
+    }
+    C2 zh = mk<T>(0, 0);
+    const bool first = (tl == 0);
+    if (first) {
+        za[0] = mk<T>(DS ? X(N - 1) : X(0), (T)0);
+        zb[0] = mk<T>(sqrt2_v<T>() * (DS ? X(L - 1) : X(L)), (T)0);
+        zh = zg(H);
+    }
+    tile_sync(wave);
+#pragma unroll
+    for (int k = 0; k < G; ++k) {
+        const int g = tl + k * TPT;
+        slot[lpad(g)] = za[k];
+        slot[lpad(L - g)] = zb[k];
+    }
+    if (first) slot[lpad(H)] = zh;
+    tile_sync(wave);
 }
 
 // two real outputs (2j, 2j+1) of a row; one 2-wide store when contiguous
@@ -330,7 +364,7 @@ __device__ __forceinline__ void post_real(const TileArgs& A, const TransformRef&
             if constexpr (KIND == K_REALDFT) {
                 oc[(long long)n * es] = z;
             } else {
-                const C2 c = tw[4 * N + n];
+                const C2 c = twl<T>(tw, (unsigned)(4 * N + n));
                 orl[(long long)re_pos(n) * es] = 2 * (z.x * c.x - z.y * c.y);
                 orl[(long long)im_pos(n) * es] = -2 * (z.x * c.y + z.y * c.x);
             }
@@ -352,7 +386,7 @@ __device__ __forceinline__ void post_real(const TileArgs& A, const TransformRef&
                 continue;
             }
             const C2 a = Tf(m), b = Tf(L - m);
-            const C2 e = tw[2 * L + m];
+            const C2 e = twl<T>(tw, (unsigned)(2 * L + m));
             const T ur = (a.x + b.x) / 2, ui = (a.y - b.y) / 2;
             const T ttr = a.x - b.x, tti = a.y + b.y;
             const T vr = (ttr * e.y + tti * e.x) / 2;  // (ttr + i tti) * e / i / 2
@@ -377,7 +411,7 @@ __device__ __forceinline__ void post_real(const TileArgs& A, const TransformRef&
                 const C2 t1 = Tf(r >> 1);
                 store_pair<T>(out, es, vec, j, (j & 1) ? t0.y : t0.x, sg * ((r & 1) ? t1.y : t1.x));
             } else {  // DCT-4 / DST-4
-                const C2 e0 = tw[8 * N + 4 * j + 1], e1 = tw[8 * N + 4 * j + 3];
+                const C2 e0 = twl<T>(tw, (unsigned)(8 * N + 4 * j + 1)), e1 = twl<T>(tw, (unsigned)(8 * N + 4 * j + 3));
                 const C2 t0 = Tf(j), t1 = Tf(L - 1 - j);
                 T y0, y1;
                 if constexpr (KIND == K_DCT4) {
@@ -421,11 +455,16 @@ __global__ void __launch_bounds__(max_threads<T>()) tile_kernel(TileArgs A) {
     const int tl = A.cfast ? tl_cf : tl_row;
     const TransformRef me = locate(A, tile * (unsigned)C + (unsigned)c);
     C2* slot = lds + (long long)c * A.sc;
+    // every exchange stays inside a wavefront: no work-group barriers
+    const bool wv = (TPT <= 64) && A.wave;
 
     C2 v[E];
 
     // ---------------- input side ----------------
-    if (A.in_mode == IO_DIRECT) {
+    // (direct input only for the complex kinds: the real kinds always stage,
+    // and compiling a dead direct path costs them registers)
+    constexpr bool CPLX = (KIND == K_DFT || KIND == K_INVDFT);
+    if (CPLX && A.in_mode == IO_DIRECT) {
         // complex input read straight into the owned positions
         const C2* __restrict__ in = reinterpret_cast<const C2*>(A.in) + me.ib;
         if (me.valid) {
@@ -450,13 +489,14 @@ __global__ void __launch_bounds__(max_threads<T>()) tile_kernel(TileArgs A) {
         const TransformRef src = (sc_ == c) ? me : locate(A, tile * (unsigned)C + (unsigned)sc_);
         C2* sslot = lds + (long long)sc_ * A.sc;
         if (src.valid) stage_in<T, KIND, L, TPT>(A, src, sslot, stl);
-        __syncthreads();
+        tile_sync(wv);
         const T* rslot = reinterpret_cast<const T*>(slot);
         auto X = [&](int i) -> T { return rslot[ridx(i)]; };
         auto Z = [&](int i) -> C2 { return slot[lpad(i)]; };
         if constexpr (KIND == K_DFT || KIND == K_INVDFT || KIND == K_REALDFT) {
             Core<T, LOGL, E, INV>::read_owned(v, slot, tl);
         } else if constexpr (E >= 2) {
+            if constexpr (KIND == K_DCT3 || KIND == K_DST3) t3_to_z<T, KIND, L, E, TPT>(slot, tl, tw, wv);
             pre_owned<T, KIND, L, E, TPT>(v, slot, tl, tw);
         } else {
 #pragma unroll
@@ -465,11 +505,11 @@ __global__ void __launch_bounds__(max_threads<T>()) tile_kernel(TileArgs A) {
                 v[m] = pre_point<T, KIND>(p, L, tw, X, Z);
             }
         }
-        __syncthreads();
+        tile_sync(wv);
     }
 
     // ---------------- FFT core ----------------
-    Core<T, LOGL, E, INV>::run(v, slot, tl, tw);
+    Core<T, LOGL, E, INV>::run(v, slot, tl, tw, wv);
 
     // two-step twiddle: W_N^{n2*p} on output point p = tl + m*TPT, computed
     // as W_N^{n2*tl} * (W_N^{n2*TPT})^m in FP64 (two table reads instead of E
@@ -508,7 +548,7 @@ __global__ void __launch_bounds__(max_threads<T>()) tile_kernel(TileArgs A) {
         }
     }
     Core<T, LOGL, E, INV>::write_owned(v, slot, tl);
-    __syncthreads();
+    tile_sync(wv);
     {
         const bool rowm = (A.out_mode != IO_STAGE_CF);
         const int sc_ = rowm ? c_row : c_cf;
