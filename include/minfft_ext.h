@@ -50,7 +50,10 @@ typedef struct minfft_batch_entry {
 typedef struct minfft_batch_plan minfft_batch_plan;
 
 /* Build a batch plan: validates entries (NULL on error), groups identical
-   shapes, builds the twiddle tables on the device and picks tile schedules. */
+   shapes, builds the twiddle tables on the device and picks each group's
+   launch schedule by timing candidates on the device (autotuner; off with
+   MINFFT_AUTOTUNE=0; results cached per shape, and in the file named by
+   MINFFT_TUNE_CACHE if set). */
 minfft_batch_plan *minfft_mkaux_batch(int count, const minfft_batch_entry *entries, int prec);
 
 /* Run the batch on device buffers `in` / `out` (device pointers) on the HIP
@@ -64,6 +67,19 @@ int minfft_batch_run(minfft_batch_plan *p, const void *in, void *out, void *stre
 int minfft_batch(minfft_batch_plan *p, const void *in, size_t in_bytes, void *out, size_t out_bytes);
 
 void minfft_free_batch(minfft_batch_plan *p);
+
+/* Replace group `group`'s schedule (groups are numbered as listed by
+   minfft_batch_describe) by the process-wide one with the "key=value,..."
+   changes in `spec` (keys tpw, cfb, wave, xcd, split, split2, fused, wpc,
+   falt;
+   NULL or "" = the process-wide schedule).  The next run re-captures the
+   graph.  Returns 0, or -1 on an unknown key or group. */
+int minfft_batch_schedule(minfft_batch_plan *p, int group, const char *spec);
+
+/* Text description of a plan's groups and their chosen schedules (one line
+   per group) into buf (at most len bytes, NUL-terminated).  Returns the
+   length needed including the NUL, or -1. */
+int minfft_batch_describe(const minfft_batch_plan *p, char *buf, int len);
 
 /* Enqueue one transform of an aux on device pointers and a caller's stream
    (hipStream_t; NULL = the aux's own stream) without waiting.  Returns 0 or -1. */

@@ -28,7 +28,7 @@ MAKERS = [f"minfft_mkaux_{fam}{suf}" for fam in ("dft", "realdft", "t2t3", "t4")
 TRANSFORMS = [f"minfft_{k}" for k in KINDS]
 SYMBOLS = TRANSFORMS + MAKERS + ["minfft_free_aux"]
 EXT_SYMBOLS = ["minfft_last_error", "minfft_mkaux_batch", "minfft_batch_run", "minfft_batch",
-               "minfft_free_batch", "minfft_enqueue"]
+               "minfft_free_batch", "minfft_batch_schedule", "minfft_batch_describe", "minfft_enqueue"]
 
 PREC_FP64, PREC_FP32, PREC_SPLIT_FP16 = 0, 1, 2
 
@@ -68,6 +68,10 @@ def engine():
                                      ctypes.c_void_p, ctypes.c_size_t]
         lib.minfft_free_batch.restype = None
         lib.minfft_free_batch.argtypes = [ctypes.c_void_p]
+        lib.minfft_batch_schedule.restype = ctypes.c_int
+        lib.minfft_batch_schedule.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_char_p]
+        lib.minfft_batch_describe.restype = ctypes.c_int
+        lib.minfft_batch_describe.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int]
         lib.minfft_enqueue.restype = ctypes.c_int
         lib.minfft_enqueue.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
                                        ctypes.c_void_p, ctypes.c_void_p]
@@ -204,6 +208,22 @@ class Batch:
         if rc:
             raise MinfftError(f"minfft_batch: {last_error()}")
         return y
+
+    def schedule(self, group, spec):
+        """Pin group `group`'s launch schedule ("key=value,..." over the
+        process-wide one; "" restores it)."""
+        rc = engine().minfft_batch_schedule(self.h, group, spec.encode())
+        if rc:
+            raise MinfftError(f"minfft_batch_schedule: {last_error()}")
+
+    def describe(self):
+        """One line per group: kind, dims, count and the chosen schedule."""
+        n = engine().minfft_batch_describe(self.h, None, 0)
+        if n < 0:
+            raise MinfftError(f"minfft_batch_describe: {last_error()}")
+        buf = ctypes.create_string_buffer(n)
+        engine().minfft_batch_describe(self.h, buf, n)
+        return buf.value.decode()
 
     def close(self):
         if self.h:

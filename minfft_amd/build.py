@@ -31,7 +31,8 @@ COMMON = [
     "-Wall", "-Wno-unused-function", "-Wno-unused-variable", "-Wno-unused-but-set-variable",
 ]
 
-HEADERS = ["fft_core.hpp", "kinds.hpp", "tile.hpp", "ktable.hpp", "engine.hpp", "big.hpp", "bigtable.hpp", "coop.hpp", "mfma.hpp"]
+HEADERS = ["fft_core.hpp", "kinds.hpp", "tile.hpp", "ktable.hpp", "engine.hpp", "big.hpp", "bigtable.hpp", "coop.hpp",
+           "mfma.hpp", "autotune.hpp", "split2.hpp"]
 
 
 def _newer(target, deps):
@@ -52,6 +53,7 @@ def _units():
     u.append((os.path.join(OBJ, "big_inst.o"), "big_inst.hip", [], "engine"))
     u.append((os.path.join(OBJ, "engine.o"), "engine.cpp", ["-x", "hip"], "engine"))
     u.append((os.path.join(OBJ, "batch.o"), "batch.cpp", ["-x", "hip"], "engine"))
+    u.append((os.path.join(OBJ, "autotune.o"), "autotune.cpp", ["-x", "hip"], "engine"))
     u.append((os.path.join(OBJ, "api64.o"), "api.cpp", ["-x", "hip"], "api64"))
     u.append((os.path.join(OBJ, "api32.o"), "api.cpp", ["-x", "hip", "-DMINFFT_SINGLE=1"], "api32"))
     return u

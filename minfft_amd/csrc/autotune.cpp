@@ -1,0 +1,237 @@
+// autotune.cpp -- picks the launch schedule of each batch group by timing
+// candidates on the device.
+//
+// A group (entries of one kind and shape) runs as a fixed sequence of axis
+// passes; what is left open is the launch schedule (Tune in engine.hpp):
+// tile widths of row and column passes, wavefront-local exchanges, XCD tile
+// order, and for long axes the single-pass radix-2-split kernel, the fused
+// kernel, its variant and occupancy, and the N = N1*N2 split of the two-step
+// decomposition.  The tuner runs the
+// group's passes on a bounded subset of its entries (at most ~512 MiB of
+// input plus output, placed contiguously in scratch buffers) and walks the
+// knobs that apply to the group one at a time (coordinate descent), keeping
+// a value when it is at least 3 % faster.  Results are cached per (precision,
+// kind, shape, count bucket) for the process, and in the file named by
+// MINFFT_TUNE_CACHE if set.  MINFFT_AUTOTUNE=0, or a MINFFT_TUNE override,
+// turns the tuner off (the process-wide schedule is used).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <mutex>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "autotune.hpp"
+
+namespace mfx {
+
+bool autotune_enabled() {
+    const char* a = std::getenv("MINFFT_AUTOTUNE");
+    if (a && std::atoi(a) == 0) return false;
+    return std::getenv("MINFFT_TUNE") == nullptr;
+}
+
+namespace {
+
+std::mutex g_mu;
+std::map<std::string, TuneResult> g_cache;
+bool g_loaded = false;
+
+std::string key_of(Prec prec, int kind, int d, const int* n, unsigned count) {
+    int cb = 0;
+    while ((1u << (cb + 1)) <= count && cb < 31) ++cb;
+    char b[128];
+    std::snprintf(b, sizeof(b), "p%d.k%d.d%d.%dx%dx%d.c%d", (int)prec, kind, d, n[0], d > 1 ? n[1] : 1,
+                  d > 2 ? n[2] : 1, cb);
+    return b;
+}
+
+// cache file line: key tpw cfb wave xcd split split2 fused wpc falt us_default us_best
+void load_cache() {
+    if (g_loaded) return;
+    g_loaded = true;
+    const char* f = std::getenv("MINFFT_TUNE_CACHE");
+    if (!f) return;
+    std::ifstream in(f);
+    std::string line;
+    while (std::getline(in, line)) {
+        std::istringstream ss(line);
+        std::string k;
+        TuneResult r;
+        r.sched = base_tune();
+        Tune& t = r.sched;
+        if (ss >> k >> t.tpw >> t.cfb >> t.wave >> t.xcd >> t.split >> t.split2 >> t.fused >> t.fused_wpc >> t.fused_alt >>
+            r.us_default >> r.us_best) {
+            r.tuned = true;
+            g_cache[k] = r;
+        }
+    }
+}
+
+void save_entry(const std::string& k, const TuneResult& r) {
+    const char* f = std::getenv("MINFFT_TUNE_CACHE");
+    if (!f) return;
+    std::ofstream out(f, std::ios::app);
+    const Tune& t = r.sched;
+    out << k << ' ' << t.tpw << ' ' << t.cfb << ' ' << t.wave << ' ' << t.xcd << ' ' << t.split << ' ' << t.split2
+        << ' ' << t.fused
+        << ' ' << t.fused_wpc << ' ' << t.fused_alt << ' ' << r.us_default << ' ' << r.us_best << '\n';
+}
+
+struct Knob {
+    int Tune::*field;
+    std::vector<int> values;
+};
+
+}  // namespace
+
+// items per entry of `kind` with dims n (complex items for complex sides)
+static void entry_items(int kind, int d, const int* n, long long& in_items, long long& out_items) {
+    long long P = 1;
+    for (int k = 0; k < d; ++k) P *= n[k];
+    const long long last = n[d - 1], rows = P / last, H = last / 2 + 1;
+    in_items = out_items = P;
+    if (kind == K_REALDFT) out_items = rows * H;
+    if (kind == K_INVREALDFT) in_items = rows * H;
+}
+
+TuneResult autotune_group(const Workspace& ws, const TuneRequest& r, hipStream_t st) {
+    TuneResult res;
+    res.sched = base_tune();
+    if (!autotune_enabled() || r.count == 0) return res;
+    const std::string key = key_of(r.prec, r.kind, r.d, r.n, r.count);
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        load_cache();
+        auto it = g_cache.find(key);
+        if (it != g_cache.end()) return it->second;
+    }
+    const size_t rs = real_size(r.prec);
+    const size_t isz = (kind_complex_in(r.kind) ? 2 : 1) * rs, osz = (kind_complex_out(r.kind) ? 2 : 1) * rs;
+    long long ii, oi;
+    entry_items(r.kind, r.d, r.n, ii, oi);
+    const size_t ebytes = (size_t)ii * isz + (size_t)oi * osz;
+    // too little work to measure reliably: keep the defaults
+    if (ebytes * r.count < ((size_t)4 << 20)) return res;
+    const unsigned cs = (unsigned)std::min<size_t>(r.count, std::max<size_t>(1, ((size_t)512 << 20) / ebytes));
+
+    int maxlog = device().maxlog[r.prec];
+    bool long_axis = false, multi = r.d > 1, short_row = r.n[r.d - 1] <= 1024;
+    for (int k = 0; k < r.d; ++k)
+        if (r.n[k] > (1 << maxlog)) long_axis = true;
+    const bool fused_ok = r.d == 1 && long_axis && (r.kind == K_DFT || r.kind == K_INVDFT) &&
+                          cs >= (unsigned)device().ncu;
+
+    std::vector<Knob> knobs;
+    if (fused_ok) {
+        knobs.push_back({&Tune::fused, {1, 0}});
+        knobs.push_back({&Tune::fused_alt, {0, 1}});
+        knobs.push_back({&Tune::fused_wpc, {2, 1, 4}});
+    }
+    if (long_axis) {
+        knobs.push_back({&Tune::split2, {1, 0}});
+        knobs.push_back({&Tune::split, {0, -1, 1}});
+    }
+    if (multi || long_axis) {
+        knobs.push_back({&Tune::cfb, {128, 64, 256}});
+        knobs.push_back({&Tune::xcd, {1, 0}});
+    }
+    knobs.push_back({&Tune::tpw, {256, 128, 512}});
+    if (short_row) knobs.push_back({&Tune::wave, {1, 0}});
+
+    void *din = nullptr, *dout = nullptr;
+    long long *dio = nullptr, *doo = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    auto cleanup = [&] {
+        if (din) (void)hipFree(din);
+        if (dout) (void)hipFree(dout);
+        if (dio) (void)hipFree(dio);
+        if (doo) (void)hipFree(doo);
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+    };
+    std::vector<long long> hio(cs), hoo(cs);
+    for (unsigned i = 0; i < cs; ++i) {
+        hio[i] = (long long)i * ii;
+        hoo[i] = (long long)i * oi;
+    }
+    if (hipMalloc(&din, (size_t)cs * ii * isz) != hipSuccess || hipMalloc(&dout, (size_t)cs * oi * osz) != hipSuccess ||
+        hipMalloc(&dio, cs * sizeof(long long)) != hipSuccess || hipMalloc(&doo, cs * sizeof(long long)) != hipSuccess ||
+        hipMemcpy(dio, hio.data(), cs * sizeof(long long), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(doo, hoo.data(), cs * sizeof(long long), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemsetAsync(din, 0, (size_t)cs * ii * isz, st) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+        hipEventCreate(&e1) != hipSuccess) {
+        (void)hipGetLastError();
+        cleanup();
+        return res;  // no room to tune: defaults
+    }
+
+    auto measure = [&](const Tune& t) -> float {
+        TuneScope scope(&t);
+        auto once = [&]() {
+            return run_entries(ws, r.prec, r.kind, r.d, r.n, din, dout, cs, dio, doo, r.d_foff, r.tbuf, r.d_toff, st);
+        };
+        if (once()) {
+            clear_error();
+            return INFINITY;
+        }
+        constexpr int reps = 3;
+        if (hipEventRecord(e0, st) != hipSuccess) return INFINITY;
+        for (int i = 0; i < reps; ++i)
+            if (once()) {
+                clear_error();
+                (void)hipStreamSynchronize(st);
+                return INFINITY;
+            }
+        if (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) return INFINITY;
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, e0, e1) != hipSuccess) return INFINITY;
+        return ms * 1000.f / reps;
+    };
+
+    Tune best = base_tune();
+    float tbest = measure(best);
+    const float tdef = tbest;
+    if (std::isfinite(tbest)) {
+        for (const Knob& k : knobs) {
+            for (int v : k.values) {
+                if (best.*(k.field) == v) continue;
+                Tune c = best;
+                c.*(k.field) = v;
+                const float t = measure(c);
+                if (t < 0.97f * tbest) {
+                    best = c;
+                    tbest = t;
+                }
+            }
+        }
+        res.sched = best;
+        res.tuned = true;
+        res.us_default = tdef;
+        res.us_best = tbest;
+    }
+    (void)hipStreamSynchronize(st);
+    cleanup();
+    if (res.tuned) {
+        std::lock_guard<std::mutex> lk(g_mu);
+        g_cache[key] = res;
+        save_entry(key, res);
+    }
+    return res;
+}
+
+std::string describe_tune(const Tune& t) {
+    char b[160];
+    std::snprintf(b, sizeof(b), "tpw=%d cfb=%d wave=%d xcd=%d split=%d split2=%d fused=%d wpc=%d falt=%d", t.tpw,
+                  t.cfb, t.wave, t.xcd, t.split, t.split2, t.fused, t.fused_wpc, t.fused_alt);
+    return b;
+}
+
+}  // namespace mfx

@@ -8,8 +8,10 @@
 // executed as the same per-axis passes as a single transform, with one extra
 // batch level whose base addresses come from device tables of per-entry
 // offsets, so a group of any size costs one launch per axis (two for an axis
-// longer than one tile).  All groups' launches are captured once into a HIP
-// graph per (in, out, stream) and replayed.
+// longer than one tile).  Each group's launch schedule is picked when the
+// plan is made by timing candidates on the device (autotune.cpp).  All
+// groups' launches are captured once into a HIP graph per (in, out, stream)
+// and replayed.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -19,6 +21,7 @@
 #include <tuple>
 #include <vector>
 
+#include "autotune.hpp"
 #include "engine.hpp"
 #include "minfft_ext.h"
 
@@ -36,6 +39,7 @@ struct BatchGroup {
     size_t tbytes = 0;     // per-group temp (inverse real DFT, d > 1)
     size_t fbytes = 0;     // per-group two-step scratch
     void* tbuf = nullptr;
+    TuneResult tune;       // launch schedule of the group's passes
 };
 
 struct minfft_batch_plan {
@@ -225,6 +229,19 @@ minfft_batch_plan* minfft_mkaux_batch(int count, const minfft_batch_entry* e, in
         p->ws.fs_bytes = fs_max;
     }
     if (hipStreamSynchronize(p->stream) != hipSuccess) return fail("twiddle generation failed");
+    for (auto& g : p->groups) {
+        TuneRequest r;
+        r.prec = p->prec;
+        r.kind = g.kind;
+        r.d = g.d;
+        std::memcpy(r.n, g.n, sizeof(r.n));
+        r.count = (unsigned)g.ioff.size();
+        r.d_foff = g.d_foff;
+        r.tbuf = g.tbuf;
+        r.d_toff = g.d_toff;
+        g.tune = autotune_group(p->ws, r, p->stream);
+    }
+    clear_error();
     return p;
 }
 
@@ -242,6 +259,7 @@ static int enqueue_batch(minfft_batch_plan* p, const void* in, void* out, hipStr
     const int nb = std::min<int>(branches, (int)p->groups.size());
     if (nb <= 1) {
         for (auto& g : p->groups) {
+            TuneScope scope(&g.tune.sched);
             int rc = run_entries(p->ws, p->prec, g.kind, g.d, g.n, in, out, (unsigned)g.ioff.size(), g.d_ioff,
                                  g.d_ooff, g.d_foff, g.tbuf, g.d_toff, st);
             if (rc) return rc;
@@ -274,6 +292,7 @@ static int enqueue_batch(minfft_batch_plan* p, const void* in, void* out, hipStr
         // the two-step scratch is shared: long groups stay on branch 0
         if (g.fbytes) b = 0;
         load[b] += group_bytes(g, p->prec);
+        TuneScope scope(&g.tune.sched);
         int rc = run_entries(p->ws, p->prec, g.kind, g.d, g.n, in, out, (unsigned)g.ioff.size(), g.d_ioff,
                              g.d_ooff, g.d_foff, g.tbuf, g.d_toff, p->side[b]);
         if (rc) return rc;
@@ -325,6 +344,58 @@ int minfft_batch_run(minfft_batch_plan* p, const void* in, void* out, void* stre
         return -1;
     }
     return 0;
+}
+
+int minfft_batch_schedule(minfft_batch_plan* p, int group, const char* spec) {
+    clear_error();
+    if (!p || group < 0 || group >= (int)p->groups.size()) {
+        set_error("invalid batch plan or group");
+        return -1;
+    }
+    TuneResult r;
+    r.sched = base_tune();
+    if (spec && *spec && !parse_tune(spec, r.sched)) {
+        set_error("unknown schedule key");
+        return -1;
+    }
+    r.tuned = false;
+    p->groups[group].tune = r;
+    // launches are recorded at capture: drop the cached graph
+    if (p->exec) (void)hipGraphExecDestroy(p->exec);
+    if (p->graph) (void)hipGraphDestroy(p->graph);
+    p->exec = nullptr;
+    p->graph = nullptr;
+    p->g_in = nullptr;
+    p->g_out = nullptr;
+    p->g_st = nullptr;
+    return 0;
+}
+
+int minfft_batch_describe(const minfft_batch_plan* p, char* buf, int len) {
+    clear_error();
+    if (!p) {
+        set_error("invalid batch plan");
+        return -1;
+    }
+    static const char* kname[K_COUNT] = {"dft", "invdft", "realdft", "invrealdft", "dct2",
+                                         "dst2", "dct3", "dst3", "dct4", "dst4"};
+    std::string s;
+    char line[320];
+    for (size_t i = 0; i < p->groups.size(); ++i) {
+        const BatchGroup& g = p->groups[i];
+        std::string dims = std::to_string(g.n[0]);
+        for (int k = 1; k < g.d; ++k) dims += "x" + std::to_string(g.n[k]);
+        std::snprintf(line, sizeof(line), "group %zu: %s %s count=%zu %s: %s default_us=%.1f tuned_us=%.1f\n", i,
+                      kname[g.kind], dims.c_str(), g.ioff.size(), g.tune.tuned ? "tuned" : "untuned",
+                      describe_tune(g.tune.sched).c_str(), g.tune.us_default, g.tune.us_best);
+        s += line;
+    }
+    if (buf && len > 0) {
+        const size_t n = std::min<size_t>(s.size(), (size_t)len - 1);
+        std::memcpy(buf, s.data(), n);
+        buf[n] = 0;
+    }
+    return (int)s.size() + 1;
 }
 
 int minfft_batch(minfft_batch_plan* p, const void* in, size_t in_bytes, void* out, size_t out_bytes) {

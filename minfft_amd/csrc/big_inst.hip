@@ -2,8 +2,8 @@
 // registry: FP32 with 512-thread work-groups, FP64 with 256 (two work-groups
 // of ~70 KiB LDS per CU where the registers allow).
 #include "big.hpp"
-#include "coop.hpp"
 #include "mfma.hpp"
+#include "split2.hpp"
 #include "bigtable.hpp"
 
 namespace mfx {
@@ -26,28 +26,23 @@ static void reg_one(BigTable& t, int prec, int variant = 0) {
     t.wpc[prec][lg] = (4 * WPE * 64) / BT;  // work-groups per CU the registers allow
 }
 
-template <typename T, int L1, int L2, int BT, int WPE>
-static void reg_coop(BigTable& t, int prec, int v) {
-    const int lg = L1 + L2;
-    t.coop[prec][lg][v][0] = reinterpret_cast<const void*>(&coop_kernel<T, L1, L2, false, BT, WPE>);
-    t.coop[prec][lg][v][1] = reinterpret_cast<const void*>(&coop_kernel<T, L1, L2, true, BT, WPE>);
-    t.coop_lds[prec][lg][v] = coop_lds_bytes<T, L1, L2, BT>();
-    t.coop_bt[prec][lg][v] = BT;
-    t.coop_wpc[prec][lg][v] = (4 * WPE * 64) / BT;
+template <typename T, int LOGH>
+static void reg_split2(BigTable& t, int prec) {
+    t.split2[prec][0] = reinterpret_cast<const void*>(&split2_kernel<T, LOGH, 16, false>);
+    t.split2[prec][1] = reinterpret_cast<const void*>(&split2_kernel<T, LOGH, 16, true>);
+    t.split2_log[prec] = LOGH + 1;
+    t.split2_bt[prec] = (1 << LOGH) / 16;
+    t.split2_lds[prec] = split2_lds_bytes<T, LOGH>();
 }
 
 void register_big(BigTable& t) {
     t.mfma512[0] = reinterpret_cast<const void*>(&mfma512_kernel<false>);
     t.mfma512[1] = reinterpret_cast<const void*>(&mfma512_kernel<true>);
-    t.coop_reset = reinterpret_cast<const void*>(&coop_reset);
-    // XCD-cooperative instances: variant 0 = 256 threads, 1 = 512 threads
-    reg_coop<float, 8, 8, 256, 4>(t, 1, 0);
-    reg_coop<float, 8, 7, 256, 4>(t, 1, 0);
-    reg_coop<float, 8, 8, 512, 4>(t, 1, 1);
-    reg_coop<float, 8, 7, 512, 4>(t, 1, 1);
-    reg_coop<double, 8, 8, 256, 2>(t, 0, 0);
-    reg_coop<double, 8, 7, 256, 2>(t, 0, 0);
-    reg_coop<double, 7, 7, 256, 2>(t, 0, 0);
+    // the XCD-cooperative kernel (coop.hpp) is not instantiated: its path is off
+    t.coop_reset = nullptr;
+    // single pass for the first length past one work-group's LDS
+    reg_split2<float, 14>(t, 1);
+    reg_split2<double, 13>(t, 0);
     // <T, log2 N1, log2 N2, threads, min waves per SIMD>
     reg_one<float, 7, 7, 512, 2>(t, 1);
     reg_one<float, 8, 7, 512, 4>(t, 1);

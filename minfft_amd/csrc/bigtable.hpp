@@ -22,6 +22,11 @@ struct BigTable {
     const void* coop_reset;
     // 512-point FP32 tiles with the split-FP16 MFMA radix-16 stage (mfma.hpp)
     const void* mfma512[2];
+    // single-pass radix-2-split kernels (split2.hpp) for L = 2^split2_log
+    const void* split2[2][2];  // [prec][inverse]
+    int split2_log[2];
+    int split2_bt[2];
+    size_t split2_lds[2];
 };
 
 void register_big(BigTable& t);

@@ -149,63 +149,63 @@ bool is_device_pointer(const void* p) {
 }
 
 // ------------------------------------------------------------ tile tuning
-// Launch-shape parameters of the tile kernel.  Defaults are the measured
-// best on MI355X; MINFFT_TUNE="key=value,..." overrides them for sweeps
-// (keys: tpw = target threads per work-group for row tiles, cfc = columns per
-// c-fast tile in bytes/8, xcd = 0|1 XCD-aware tile order, stage = 0|1 force
-// LDS-staged complex I/O).
-struct Tune {
-    int tpw = 256;
-    int cfb = 128;  // bytes per row segment of a c-fast tile
-    int wave = 1;   // wavefront-local exchanges for row tiles of <= 64 lanes per transform
-    int xcd = 1;
-    int stage = 0;
-    int chunk_mb = 64;  // two-step chunk size for entry batches
-    int fused = 1;      // fused two-step kernel for batches of long 1D entries
-    int fused_wpc = 2;  // work-groups per CU of the fused kernel
-    int fused_alt = 0;  // use the alternative (smaller) fused work-groups
-    int coop = 0;       // XCD-cooperative kernel (coop.hpp): off and not selectable --
-                        // two GPU runs hung (see the header of coop.hpp)
-    int coop_v = 0;     // its launch-shape variant
-    int coop_wpc = 4;   // its work-groups per CU
-    int coop_la = 3;    // queue lookahead (rounds)
-    int coop_s = 4;     // scratch slots per XCD (> lookahead)
-};
+// Process-wide schedule: defaults, then MINFFT_TUNE="key=value,..." (keys:
+// tpw, cfb, wave, xcd, stage, chunk, split, fused, wpc, falt, cv, cwpc, cla,
+// cs -- the fields of Tune in engine.hpp).
 static Tune g_tune;
 static std::once_flag g_tune_once;
-static const Tune& tune() {
+static thread_local const Tune* t_tune = nullptr;
+
+bool parse_tune(const char* spec, Tune& t) {
+    std::string s(spec ? spec : "");
+    size_t pos = 0;
+    bool ok = true;
+    while (pos < s.size()) {
+        size_t c = s.find(',', pos);
+        if (c == std::string::npos) c = s.size();
+        std::string kv = s.substr(pos, c - pos);
+        pos = c + 1;
+        if (kv.empty()) continue;
+        size_t eq = kv.find('=');
+        if (eq == std::string::npos) {
+            ok = false;
+            continue;
+        }
+        std::string k = kv.substr(0, eq);
+        int v = std::atoi(kv.c_str() + eq + 1);
+        if (k == "tpw" && v > 0) t.tpw = v;
+        else if (k == "cfb" && v > 0) t.cfb = v;
+        else if (k == "wave") t.wave = v;
+        else if (k == "xcd") t.xcd = v;
+        else if (k == "stage") t.stage = v;
+        else if (k == "chunk" && v > 0) t.chunk_mb = v;
+        else if (k == "split") t.split = v;
+        else if (k == "split2") t.split2 = v;
+        else if (k == "fused") t.fused = v;
+        else if (k == "wpc" && v > 0) t.fused_wpc = v;
+        else if (k == "falt") t.fused_alt = v;
+        else if (k == "cv") t.coop_v = v;
+        else if (k == "cwpc" && v > 0) t.coop_wpc = v;
+        else if (k == "cla" && v >= 0) t.coop_la = v;
+        else if (k == "cs" && v > 0) t.coop_s = v;
+        else ok = false;
+    }
+    t.coop = 0;  // never selectable
+    return ok;
+}
+
+const Tune& base_tune() {
     std::call_once(g_tune_once, [] {
         const char* e = std::getenv("MINFFT_TUNE");
-        if (!e) return;
-        std::string s(e);
-        size_t pos = 0;
-        while (pos < s.size()) {
-            size_t c = s.find(',', pos);
-            if (c == std::string::npos) c = s.size();
-            std::string kv = s.substr(pos, c - pos);
-            size_t eq = kv.find('=');
-            if (eq != std::string::npos) {
-                std::string k = kv.substr(0, eq);
-                int v = std::atoi(kv.c_str() + eq + 1);
-                if (k == "tpw" && v > 0) g_tune.tpw = v;
-                else if (k == "cfb" && v > 0) g_tune.cfb = v;
-                else if (k == "wave") g_tune.wave = v;
-                else if (k == "xcd") g_tune.xcd = v;
-                else if (k == "stage") g_tune.stage = v;
-                else if (k == "chunk" && v > 0) g_tune.chunk_mb = v;
-                else if (k == "fused") g_tune.fused = v;
-                else if (k == "wpc" && v > 0) g_tune.fused_wpc = v;
-                else if (k == "falt") g_tune.fused_alt = v;
-                else if (k == "cv") g_tune.coop_v = v;
-                else if (k == "cwpc" && v > 0) g_tune.coop_wpc = v;
-                else if (k == "cla" && v >= 0) g_tune.coop_la = v;
-                else if (k == "cs" && v > 0) g_tune.coop_s = v;
-            }
-            pos = c + 1;
-        }
+        if (e) (void)parse_tune(e, g_tune);
     });
     return g_tune;
 }
+
+const Tune& tune() { return t_tune ? *t_tune : base_tune(); }
+
+TuneScope::TuneScope(const Tune* t) : prev(t_tune) { t_tune = t; }
+TuneScope::~TuneScope() { t_tune = prev; }
 
 // -------------------------------------------------------------- launching
 static std::mutex g_attr_mu;
@@ -366,12 +366,44 @@ static int c2c_axis(const Workspace& ws, Prec prec, int kind, int logL, const vo
     const int maxlog = device().maxlog[prec];
     if (logL <= maxlog)
         return launch_tile(prec, kind, logL, ws.tw, ws.tw64, in, in_e, out, out_e, lv, -1, 0, st, ws.split16 != 0);
+    // one past the tile limit, contiguous: single pass (split2.hpp)
+    {
+        const BigTable& bt = bigtable();
+        const int pi = prec == PREC_F64 ? 0 : 1;
+        if (tune().split2 && in_e == 1 && out_e == 1 && bt.split2[pi][0] && bt.split2_log[pi] == logL &&
+            bt.split2_lds[pi] <= (size_t)device().lds_max) {
+            TileArgs A;
+            std::memset(&A, 0, sizeof(A));
+            A.in = in;
+            A.out = out;
+            A.tw = ws.tw;
+            A.tw64 = (const double2*)ws.tw64;
+            A.in_e = 1;
+            A.out_e = 1;
+            fill_levels(A, lv);
+            const unsigned long long ntr = lv.count();
+            if (ntr == 0) return 0;
+            if (ntr > 0x7fffffffull) {
+                set_error("batch too large");
+                return -1;
+            }
+            A.ntrans = (unsigned)ntr;
+            A.ntiles = (unsigned)ntr;
+            A.C = 1;
+            A.xcd = 1;
+            void* args[] = {&A};
+            return launch_fn(bt.split2[pi][kind == K_INVDFT], (unsigned)ntr, (unsigned)bt.split2_bt[pi], args,
+                             bt.split2_lds[pi], st);
+        }
+    }
     if (in_e != out_e) {
         set_error("internal: two-step pass needs equal strides");
         return -1;
     }
     const long long e = in_e;
-    const int l1 = logL / 2, l2 = logL - l1;
+    // N = N1 * N2; the schedule may move the split off the middle
+    const int l1 = std::max(std::max(1, logL - maxlog), std::min(logL / 2 + tune().split, std::min(maxlog, logL - 1)));
+    const int l2 = logL - l1;
     if (l2 > maxlog) {
         set_error("transform length exceeds the two-step limit");
         return -1;
@@ -553,10 +585,9 @@ int axis_pass(const Workspace& ws, Prec prec, int kind, int logN, const void* in
         set_error("internal: missing two-step scratch");
         return -1;
     }
-    if (lv.ioff || lv.ooff) {
-        set_error("long real transforms are not supported in batches");
-        return -1;
-    }
+    // (with entry tables: the pre-kernel reads through the input table and
+    // writes the compact scratch, the post-kernel writes through the output
+    // table; the two-step FFT runs on the compact scratch)
     const int L = 1 << logL;
     const unsigned long long ntr = lv.count();
     TileArgs A;
@@ -567,6 +598,7 @@ int axis_pass(const Workspace& ws, Prec prec, int kind, int logN, const void* in
     A.in = in;
     A.in_e = in_e;
     A.out = ws.gbuf;
+    A.out_off = nullptr;
     int rc = launch_elementwise(kt.pre[kind], A, L, (long long)ntr * L, st);
     if (rc) return rc;
     Levels g;
@@ -580,6 +612,7 @@ int axis_pass(const Workspace& ws, Prec prec, int kind, int logN, const void* in
     B.ntrans = (unsigned)ntr;
     B.tw = ws.tw;
     B.in = ws.gbuf;
+    B.in_off = nullptr;
     B.out = out;
     B.out_e = out_e;
     return launch_elementwise(kt.post[kind], B, L, (long long)ntr * kind_nout(kind, L), st);

@@ -78,6 +78,45 @@ struct Plan {
     bool own_stream = true;
 };
 
+// Launch schedule of the passes.  The defaults are the measured best for
+// large batches on MI355X; MINFFT_TUNE="key=value,..." overrides them for a
+// process, and batch plans pick one per group with the autotuner
+// (autotune.cpp), installed for the group's launches with TuneScope.
+struct Tune {
+    int tpw = 256;      // target threads per work-group of row tiles
+    int cfb = 128;      // bytes per row segment of a column (c-fast) tile
+    int wave = 1;       // wavefront-local exchanges for row tiles of <= 64 lanes per transform
+    int xcd = 1;        // XCD-aware tile order for column tiles
+    int stage = 0;      // force LDS-staged complex I/O
+    int chunk_mb = 64;  // two-step chunk size for entry batches
+    int split = 0;      // two-step split of a long axis: log2 N1 = log2(N)/2 + split
+    int split2 = 1;     // single-pass radix-2-split kernel for the first length past one tile
+    int fused = 1;      // fused two-step kernel for batches of long 1D entries
+    int fused_wpc = 2;  // work-groups per CU of the fused kernel
+    int fused_alt = 0;  // use the alternative (smaller) fused work-groups
+    int coop = 0;       // XCD-cooperative kernel (coop.hpp): off and not selectable --
+                        // two GPU runs hung (see the header of coop.hpp)
+    int coop_v = 0;     // its launch-shape variant
+    int coop_wpc = 4;   // its work-groups per CU
+    int coop_la = 3;    // queue lookahead (rounds)
+    int coop_s = 4;     // scratch slots per XCD (> lookahead)
+};
+// the schedule in force on this thread
+const Tune& tune();
+// applies "key=value,..." (the Tune fields: tpw, cfb, wave, xcd, stage,
+// chunk, split, split2, fused, wpc, falt) to t; false on an unknown key
+bool parse_tune(const char* spec, Tune& t);
+// the process-wide schedule (defaults + MINFFT_TUNE)
+const Tune& base_tune();
+// installs `t` for the launches issued by this thread while alive
+struct TuneScope {
+    explicit TuneScope(const Tune* t);
+    ~TuneScope();
+    TuneScope(const TuneScope&) = delete;
+    TuneScope& operator=(const TuneScope&) = delete;
+    const Tune* prev;
+};
+
 // error channel (thread-local)
 void set_error(const std::string& msg);
 const char* last_error();

@@ -1,0 +1,63 @@
+// split2.hpp -- single-pass complex FFT of length L = 2H, one work-group
+// per transform, for the first length beyond one work-group's LDS (FP32
+// L = 2^15, FP64 L = 2^14): one HBM read and one HBM write per point
+// instead of the two of the two-step decomposition.
+//
+// Radix-2 decimation in frequency:
+//     a[n] = x[n] + x[n+H],   b[n] = (x[n] - x[n+H]) * W_L^n    (n < H)
+//     X[2k] = DFT_H(a)[k],    X[2k+1] = DFT_H(b)[k]
+// (inverse: conjugate twiddle, inverse DFT_H).  Each lane loads x at its
+// owned positions n = tl + m*TPT of both halves, forms a and b in
+// registers, runs DFT_H(a) through the LDS slot while b waits in registers,
+// then DFT_H(b) while DFT_H(a)'s result waits, and writes the pairs
+// (X[2k], X[2k+1]) of its owned k -- adjacent lanes, adjacent pairs.
+// Contiguous transforms only (item stride 1 on both sides).
+#pragma once
+#include "tile.hpp"
+
+namespace mfx {
+
+template <typename T, int LOGH, int E, bool INV>
+__global__ void __launch_bounds__((1 << LOGH) / E) split2_kernel(TileArgs A) {
+    using C2 = c2<T>;
+    constexpr int H = 1 << LOGH, L = 2 * H, TPT = H / E;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    C2* slot = reinterpret_cast<C2*>(smem_raw);
+    const C2* __restrict__ tw = reinterpret_cast<const C2*>(A.tw);
+    const int tl = threadIdx.x;
+    // one transform per work-group; the grid is exactly the transform count
+    const TransformRef me = locate(A, blockIdx.x);
+    if (!me.valid) return;  // uniform over the work-group
+    const C2* __restrict__ in = reinterpret_cast<const C2*>(A.in) + me.ib;
+    C2 v[E], w[E];
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+        v[m] = in[tl + m * TPT];
+        w[m] = in[tl + m * TPT + H];
+    }
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+        const C2 e = twl<T>(tw, (unsigned)(L + tl + m * TPT));  // W_L^n
+        const C2 s = add<T>(v[m], w[m]);
+        const C2 d = sub<T>(v[m], w[m]);
+        v[m] = s;
+        w[m] = INV ? cmulc<T>(d, e) : cmul<T>(d, e);
+    }
+    Core<T, LOGH, E, INV>::run(v, slot, tl, tw);  // v: X[2k] at k = tl + m*TPT
+    __syncthreads();                              // slot free for the second half
+    Core<T, LOGH, E, INV>::run(w, slot, tl, tw);  // w: X[2k+1]
+    C2* __restrict__ out = reinterpret_cast<C2*>(A.out) + me.ob;
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+        const int k = tl + m * TPT;
+        out[2 * k] = v[m];
+        out[2 * k + 1] = w[m];
+    }
+}
+
+template <typename T, int LOGH>
+constexpr size_t split2_lds_bytes() {
+    return (size_t)slot_len(1 << LOGH) * sizeof(c2<T>);
+}
+
+}  // namespace mfx

@@ -1,0 +1,134 @@
+"""GPU tests of the batch autotuner and of pinned schedules: every schedule
+the tuner can choose must give the same results (to the parity bars) as the
+default one, i.e. agree with the CPU oracle."""
+import numpy as np
+import pytest
+
+import minfft_amd as mf
+from oracle.oracle import out_shape, is_complex_in, is_complex_out
+from tests.helpers import TOL32, make_input, rel_l2, tol64
+
+pytestmark = pytest.mark.gpu
+
+# every knob value the tuner tries (autotune.cpp), alone and in the
+# combinations that change the long-axis path
+SPECS = ["", "tpw=128", "tpw=512", "cfb=64", "cfb=256", "xcd=0", "wave=0",
+         "fused=0", "falt=1", "wpc=1", "wpc=4", "split=-1", "split=1",
+         "split2=0", "split2=0,split=-1", "split2=0,split=1",
+         "fused=0,split=-1", "fused=0,split=1"]
+
+
+class Packed:
+    """Entries packed into one device input and one device output buffer."""
+
+    def __init__(self, entries, xs, prec):
+        import torch
+        self.torch = torch
+        self.prec = prec
+        rs = 8 if prec == 64 else 4
+        self.rs = rs
+        ioffs, ooffs = [], []
+        ipos = opos = 0
+        for (kind, shape), x in zip(entries, xs):
+            ioffs.append(ipos)
+            ipos += x.nbytes + 16 * rs
+            n_out = int(np.prod(out_shape(kind, shape))) * (2 if is_complex_out(kind) else 1)
+            ooffs.append(opos)
+            opos += n_out * rs + 16 * rs
+        hin = np.zeros(ipos, dtype=np.uint8)
+        for x, o in zip(xs, ioffs):
+            hin[o:o + x.nbytes] = np.frombuffer(np.ascontiguousarray(x).tobytes(), dtype=np.uint8)
+        self.din = torch.from_numpy(hin).cuda()
+        self.dout = torch.zeros(opos, dtype=torch.uint8, device="cuda")
+        self.entries, self.xs, self.ooffs = entries, xs, ooffs
+        self.batch = mf.Batch([(k, s, io, oo) for (k, s), io, oo in zip(entries, ioffs, ooffs)],
+                              mf.PREC_FP64 if prec == 64 else mf.PREC_FP32)
+
+    def run(self):
+        self.dout.zero_()
+        self.batch.run(self.din.data_ptr(), self.dout.data_ptr(), None)
+        self.torch.cuda.synchronize()
+        return self.dout.cpu().numpy()
+
+    def output(self, hout, i):
+        kind, shape = self.entries[i]
+        rdt = np.float64 if self.prec == 64 else np.float32
+        cdt = np.complex128 if self.prec == 64 else np.complex64
+        odt = cdt if is_complex_out(kind) else rdt
+        n_out = int(np.prod(out_shape(kind, shape)))
+        o = self.ooffs[i]
+        return hout[o:o + n_out * np.dtype(odt).itemsize].view(odt).reshape(out_shape(kind, shape))
+
+
+# the longest single-tile length + 1 (FP32 2^15, FP64 2^14): single-pass
+# radix-2-split kernel or two-step; + 2: fused two-step kernel (>= 256
+# entries) or two-step
+def _long(prec):
+    return (1 << 16, 1 << 15) if prec == 32 else (1 << 15, 1 << 14)
+
+
+def _cases(prec):
+    rng = np.random.default_rng(77 + prec)
+    fused_n, split_n = _long(prec)
+    entries = []
+    entries += [("dft", (64, 256))] * 24 + [("invdft", (128, 32, 16))] * 4
+    entries += [("dft", (fused_n,))] * 260 + [("invdft", (split_n,))] * 40
+    entries += [("dct2", (128, 128))] * 8 + [("dst4", (64, 32))] * 8
+    entries += [("realdft", (1 << 17,))] * 3 + [("realdft", (2 * split_n,))] * 2
+    xs = [make_input(k, s, prec, int(rng.integers(1 << 30))) for k, s in entries]
+    return entries, xs
+
+
+@pytest.mark.parametrize("prec", [32, 64])
+def test_every_schedule_matches_oracle(oracle64, prec):
+    entries, xs = _cases(prec)
+    pk = Packed(entries, xs, prec)
+    ngroups = pk.batch.describe().count("\n")
+    # entries to check: all but the fused-length ones, of which a spread
+    long_idx = [i for i, (k, s) in enumerate(entries) if s == (_long(prec)[0],)]
+    check = [i for i in range(len(entries)) if i not in long_idx] + long_idx[::41] + [long_idx[-1]]
+    refs = {}
+    for i in check:
+        kind, shape = entries[i]
+        x = xs[i].astype(np.complex128 if is_complex_in(kind) else np.float64)
+        refs[i] = oracle64.run(kind, x, shape)
+    for spec in SPECS:
+        for g in range(ngroups):
+            pk.batch.schedule(g, spec)
+        print(f"[case] schedule '{spec}' prec={prec}", flush=True)
+        hout = pk.run()
+        for i in check:
+            kind, shape = entries[i]
+            bar = tol64(shape) if prec == 64 else TOL32
+            e = rel_l2(pk.output(hout, i), refs[i])
+            assert e <= bar, (spec, i, kind, shape, e)
+    pk.batch.close()
+
+
+def test_autotuner_tunes_large_groups(oracle64):
+    """A group with enough work is timed at plan time; its schedule is
+    reported and the results agree with the oracle."""
+    n = 1 << 12
+    cnt = 2048
+    entries = [("dft", (n,))] * cnt + [("dct2", (16,))] * 3
+    rng = np.random.default_rng(5)
+    xs = [make_input(k, s, 32, int(rng.integers(1 << 30))) for k, s in entries]
+    pk = Packed(entries, xs, 32)
+    desc = pk.batch.describe()
+    print(desc)
+    lines = desc.strip().split("\n")
+    assert len(lines) == 2
+    big = [ln for ln in lines if f"dft {n} count={cnt}" in ln]
+    assert big and " tuned:" in big[0], desc
+    small = [ln for ln in lines if "dct2 16 count=3" in ln]
+    assert small and "untuned" in small[0], desc
+    hout = pk.run()
+    for i in [0, 1, 777, cnt - 1, cnt, cnt + 2]:
+        kind, shape = entries[i]
+        ref = oracle64.run(kind, xs[i].astype(np.complex128 if is_complex_in(kind) else np.float64), shape)
+        assert rel_l2(pk.output(hout, i), ref) <= TOL32
+    with pytest.raises(mf.MinfftError):
+        pk.batch.schedule(0, "nonsense=3")
+    with pytest.raises(mf.MinfftError):
+        pk.batch.schedule(99, "")
+    pk.batch.close()
