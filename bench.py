@@ -478,6 +478,8 @@ def main():
 
     ms32, xin, xout, offs, plan = gpu_batch(sizes, mf.PREC_FP32, a.steps, a.warmup, torch, mf)
     log(f"[bench] fp32 batch: {ms32:.4f} ms/step")
+    schedule = [ln.split(": ", 1)[1] for ln in plan.describe().strip().split("\n") if ": " in ln]
+    log("[bench] schedule:\n  " + "\n  ".join(schedule))
     acc = check_sample(xin, xout, sizes, offs)
     log(f"[bench] accuracy: {acc}")
     plan.close()
@@ -527,6 +529,8 @@ def main():
                      "algorithmic_bytes": dom_bytes, "ms_per_launch": round(dom_ms, 5)},
         "step_roofline": {"achieved": round(step_achieved, 2), "frac": round(step_achieved / HBM_PEAK_GBS, 4),
                           "algorithmic_bytes": alg_bytes, "what": "whole batch step, all lengths, one graph launch"},
+        "schedule": {"autotuned": os.environ.get("MINFFT_AUTOTUNE", "1") != "0" and "MINFFT_TUNE" not in os.environ,
+                     "groups": schedule},
         "accuracy_vs_fp64_oracle": acc,
         "fp64_gpu_same_workload": fp64,
         "speedup_vs_fp64_gpu": round(fp64["ms_per_step"] / ms32, 3) if fp64 else None,

@@ -3,8 +3,8 @@
 //
 // A group (entries of one kind and shape) runs as a fixed sequence of axis
 // passes; what is left open is the launch schedule (Tune in engine.hpp):
-// tile widths of row and column passes, wavefront-local exchanges, XCD tile
-// order, and for long axes the single-pass radix-2-split kernel, the fused
+// tile widths of row and column passes, the core mapping of column passes,
+// wavefront-local exchanges, XCD tile order, and for long axes the single-pass radix-2-split kernel, the fused
 // kernel, its variant and occupancy, and the N = N1*N2 split of the two-step
 // decomposition.  The tuner runs the
 // group's passes on a bounded subset of its entries (at most ~512 MiB of
@@ -53,7 +53,7 @@ std::string key_of(Prec prec, int kind, int d, const int* n, unsigned count) {
     return b;
 }
 
-// cache file line: key tpw cfb wave xcd split split2 fused wpc falt us_default us_best
+// cache file line: key tpw cfb cfcore wave xcd split split2 fused wpc falt us_default us_best
 void load_cache() {
     if (g_loaded) return;
     g_loaded = true;
@@ -67,7 +67,8 @@ void load_cache() {
         TuneResult r;
         r.sched = base_tune();
         Tune& t = r.sched;
-        if (ss >> k >> t.tpw >> t.cfb >> t.wave >> t.xcd >> t.split >> t.split2 >> t.fused >> t.fused_wpc >> t.fused_alt >>
+        if (ss >> k >> t.tpw >> t.cfb >> t.cfcore >> t.wave >> t.xcd >> t.split >> t.split2 >> t.fused >>
+            t.fused_wpc >> t.fused_alt >>
             r.us_default >> r.us_best) {
             r.tuned = true;
             g_cache[k] = r;
@@ -80,7 +81,8 @@ void save_entry(const std::string& k, const TuneResult& r) {
     if (!f) return;
     std::ofstream out(f, std::ios::app);
     const Tune& t = r.sched;
-    out << k << ' ' << t.tpw << ' ' << t.cfb << ' ' << t.wave << ' ' << t.xcd << ' ' << t.split << ' ' << t.split2
+    out << k << ' ' << t.tpw << ' ' << t.cfb << ' ' << t.cfcore << ' ' << t.wave << ' ' << t.xcd << ' ' << t.split
+        << ' ' << t.split2
         << ' ' << t.fused
         << ' ' << t.fused_wpc << ' ' << t.fused_alt << ' ' << r.us_default << ' ' << r.us_best << '\n';
 }
@@ -123,9 +125,11 @@ TuneResult autotune_group(const Workspace& ws, const TuneRequest& r, hipStream_t
     const unsigned cs = (unsigned)std::min<size_t>(r.count, std::max<size_t>(1, ((size_t)512 << 20) / ebytes));
 
     int maxlog = device().maxlog[r.prec];
-    bool long_axis = false, multi = r.d > 1, short_row = r.n[r.d - 1] <= 1024;
-    for (int k = 0; k < r.d; ++k)
+    bool long_axis = false, near_limit = false, multi = r.d > 1, short_row = r.n[r.d - 1] <= 1024;
+    for (int k = 0; k < r.d; ++k) {
         if (r.n[k] > (1 << maxlog)) long_axis = true;
+        if (r.n[k] >= (1 << (maxlog - 1))) near_limit = true;
+    }
     const bool fused_ok = r.d == 1 && long_axis && (r.kind == K_DFT || r.kind == K_INVDFT) &&
                           cs >= (unsigned)device().ncu;
 
@@ -135,11 +139,10 @@ TuneResult autotune_group(const Workspace& ws, const TuneRequest& r, hipStream_t
         knobs.push_back({&Tune::fused_alt, {0, 1}});
         knobs.push_back({&Tune::fused_wpc, {2, 1, 4}});
     }
-    if (long_axis) {
-        knobs.push_back({&Tune::split2, {1, 0}});
-        knobs.push_back({&Tune::split, {0, -1, 1}});
-    }
+    if (near_limit) knobs.push_back({&Tune::split2, {-1, 1, 2, 3, 0}});
+    if (long_axis) knobs.push_back({&Tune::split, {0, -1, 1}});
     if (multi || long_axis) {
+        knobs.push_back({&Tune::cfcore, {-1, 0, 1}});
         knobs.push_back({&Tune::cfb, {128, 64, 256}});
         knobs.push_back({&Tune::xcd, {1, 0}});
     }
@@ -229,8 +232,8 @@ TuneResult autotune_group(const Workspace& ws, const TuneRequest& r, hipStream_t
 
 std::string describe_tune(const Tune& t) {
     char b[160];
-    std::snprintf(b, sizeof(b), "tpw=%d cfb=%d wave=%d xcd=%d split=%d split2=%d fused=%d wpc=%d falt=%d", t.tpw,
-                  t.cfb, t.wave, t.xcd, t.split, t.split2, t.fused, t.fused_wpc, t.fused_alt);
+    std::snprintf(b, sizeof(b), "tpw=%d cfb=%d cfcore=%d wave=%d xcd=%d split=%d split2=%d fused=%d wpc=%d falt=%d",
+                  t.tpw, t.cfb, t.cfcore, t.wave, t.xcd, t.split, t.split2, t.fused, t.fused_wpc, t.fused_alt);
     return b;
 }
 

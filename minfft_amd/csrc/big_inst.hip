@@ -26,13 +26,12 @@ static void reg_one(BigTable& t, int prec, int variant = 0) {
     t.wpc[prec][lg] = (4 * WPE * 64) / BT;  // work-groups per CU the registers allow
 }
 
-template <typename T, int LOGH>
+template <typename T, int LOGH, int WPE>
 static void reg_split2(BigTable& t, int prec) {
-    t.split2[prec][0] = reinterpret_cast<const void*>(&split2_kernel<T, LOGH, 16, false>);
-    t.split2[prec][1] = reinterpret_cast<const void*>(&split2_kernel<T, LOGH, 16, true>);
-    t.split2_log[prec] = LOGH + 1;
-    t.split2_bt[prec] = (1 << LOGH) / 16;
-    t.split2_lds[prec] = split2_lds_bytes<T, LOGH>();
+    t.split2[prec][LOGH + 1][0] = reinterpret_cast<const void*>(&split2_kernel<T, LOGH, 16, false, WPE>);
+    t.split2[prec][LOGH + 1][1] = reinterpret_cast<const void*>(&split2_kernel<T, LOGH, 16, true, WPE>);
+    t.split2_bt[prec][LOGH + 1] = (1 << LOGH) / 16;
+    t.split2_lds[prec][LOGH + 1] = split2_lds_bytes<T, LOGH>();
 }
 
 void register_big(BigTable& t) {
@@ -40,9 +39,14 @@ void register_big(BigTable& t) {
     t.mfma512[1] = reinterpret_cast<const void*>(&mfma512_kernel<true>);
     // the XCD-cooperative kernel (coop.hpp) is not instantiated: its path is off
     t.coop_reset = nullptr;
-    // single pass for the first length past one work-group's LDS
-    reg_split2<float, 14>(t, 1);
-    reg_split2<double, 13>(t, 0);
+    // <T, log2 H, min waves per SIMD>: the first length past one
+    // work-group's LDS, and the two lengths below it (two work-groups per CU)
+    reg_split2<float, 14, 4>(t, 1);
+    reg_split2<float, 13, 4>(t, 1);
+    reg_split2<float, 12, 3>(t, 1);
+    reg_split2<double, 13, 2>(t, 0);
+    reg_split2<double, 12, 2>(t, 0);
+    reg_split2<double, 11, 2>(t, 0);
     // <T, log2 N1, log2 N2, threads, min waves per SIMD>
     reg_one<float, 7, 7, 512, 2>(t, 1);
     reg_one<float, 8, 7, 512, 4>(t, 1);

@@ -22,11 +22,10 @@ struct BigTable {
     const void* coop_reset;
     // 512-point FP32 tiles with the split-FP16 MFMA radix-16 stage (mfma.hpp)
     const void* mfma512[2];
-    // single-pass radix-2-split kernels (split2.hpp) for L = 2^split2_log
-    const void* split2[2][2];  // [prec][inverse]
-    int split2_log[2];
-    int split2_bt[2];
-    size_t split2_lds[2];
+    // single-pass radix-2-split kernels (split2.hpp), by log2 L
+    const void* split2[2][17][2];  // [prec][log2 L][inverse]
+    int split2_bt[2][17];
+    size_t split2_lds[2][17];
 };
 
 void register_big(BigTable& t);

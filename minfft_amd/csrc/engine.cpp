@@ -181,6 +181,7 @@ bool parse_tune(const char* spec, Tune& t) {
         else if (k == "chunk" && v > 0) t.chunk_mb = v;
         else if (k == "split") t.split = v;
         else if (k == "split2") t.split2 = v;
+        else if (k == "cfcore") t.cfcore = v;
         else if (k == "fused") t.fused = v;
         else if (k == "wpc" && v > 0) t.fused_wpc = v;
         else if (k == "falt") t.fused_alt = v;
@@ -287,20 +288,27 @@ static int launch_tile(Prec prec, int kind, int logL, const void* tw, const void
     const bool out_cf = f >= 0 && !tab_f && A.out_s[f] == 1;
     const bool cplx = (kind == K_DFT || kind == K_INVDFT);
     const Tune& tn = tune();
+    // core mapping of column passes: c-fast (complex kinds: direct I/O) or
+    // row-mapped with the column I/O staged through LDS
+    const bool cf_core = tn.cfcore < 0 ? cplx : tn.cfcore != 0;
     if (cplx && tn.stage) {
         A.in_mode = in_row ? IO_STAGE_ROW : IO_STAGE_CF;
         A.out_mode = out_row ? IO_STAGE_ROW : IO_STAGE_CF;
-        A.cfast = (A.in_mode == IO_STAGE_CF) ? 1 : 0;
-    } else if (cplx) {
+        A.cfast = (A.in_mode == IO_STAGE_CF && cf_core) ? 1 : 0;
+    } else if (cplx && cf_core) {
         if (in_row && out_row) { A.cfast = 0; A.in_mode = IO_DIRECT; A.out_mode = IO_DIRECT; }
         else if (in_cf && out_cf) { A.cfast = 1; A.in_mode = IO_DIRECT; A.out_mode = IO_DIRECT; }
         else if (out_cf) { A.cfast = 1; A.out_mode = IO_DIRECT; A.in_mode = in_row ? IO_STAGE_ROW : IO_DIRECT; }
         else if (out_row) { A.cfast = 0; A.out_mode = IO_DIRECT; A.in_mode = in_cf ? IO_STAGE_CF : IO_DIRECT; }
         else { A.cfast = in_cf ? 1 : 0; A.in_mode = IO_DIRECT; A.out_mode = IO_DIRECT; }
+    } else if (cplx) {
+        A.cfast = 0;
+        A.in_mode = in_row ? IO_DIRECT : (in_cf ? IO_STAGE_CF : IO_DIRECT);
+        A.out_mode = out_row ? IO_DIRECT : (out_cf ? IO_STAGE_CF : IO_DIRECT);
     } else {
         A.in_mode = in_row ? IO_STAGE_ROW : (in_cf ? IO_STAGE_CF : IO_STAGE_ROW);
         A.out_mode = out_row ? IO_STAGE_ROW : (out_cf ? IO_STAGE_CF : IO_STAGE_ROW);
-        A.cfast = (A.in_mode == IO_STAGE_CF) ? 1 : 0;
+        A.cfast = (A.in_mode == IO_STAGE_CF && cf_core) ? 1 : 0;
     }
     const bool wants_cf = A.cfast || A.in_mode == IO_STAGE_CF || A.out_mode == IO_STAGE_CF;
     const size_t cs = prec == PREC_F64 ? 16 : 8;
@@ -329,7 +337,9 @@ static int launch_tile(Prec prec, int kind, int logL, const void* tw, const void
     const unsigned long long tiles = (ntr + C - 1) / C;
     A.ntiles = (unsigned)tiles;
     A.xcd = (tn.xcd && wants_cf && tiles >= 2ull * D.nxcd) ? D.nxcd : 1;
-    A.wave = (tn.wave && !wants_cf) ? 1 : 0;
+    // wavefront-local core exchanges need a row-mapped core; staged column
+    // I/O keeps its work-group barriers (tile.hpp)
+    A.wave = (tn.wave && !A.cfast) ? 1 : 0;
     void* args[] = {&A};
     return launch_fn(fn, (unsigned)tiles, (unsigned)(C * TPT), args, (size_t)C * sc * cs, st);
 }
@@ -361,41 +371,51 @@ size_t axis_scratch_bytes(Prec prec, int kind, int logN, unsigned long long ntra
 }
 
 // complex pass along an axis of element stride e (same on both sides)
+// single-pass radix-2-split launch (split2.hpp) for contiguous transforms of
+// length 2^logL, if instantiated and its LDS fits; 1 = not applicable
+static int launch_split2(const Workspace& ws, Prec prec, int kind, int logL, const void* in, long long in_e,
+                         void* out, long long out_e, const Levels& lv, hipStream_t st) {
+    const BigTable& bt = bigtable();
+    const int pi = prec == PREC_F64 ? 0 : 1;
+    if (in_e != 1 || out_e != 1 || logL > 16 || !bt.split2[pi][logL][0] ||
+        bt.split2_lds[pi][logL] > (size_t)device().lds_max)
+        return 1;
+    TileArgs A;
+    std::memset(&A, 0, sizeof(A));
+    A.in = in;
+    A.out = out;
+    A.tw = ws.tw;
+    A.tw64 = (const double2*)ws.tw64;
+    A.in_e = 1;
+    A.out_e = 1;
+    fill_levels(A, lv);
+    const unsigned long long ntr = lv.count();
+    if (ntr == 0) return 0;
+    if (ntr > 0x7fffffffull) {
+        set_error("batch too large");
+        return -1;
+    }
+    A.ntrans = (unsigned)ntr;
+    A.ntiles = (unsigned)ntr;
+    A.C = 1;
+    A.xcd = 1;
+    void* args[] = {&A};
+    return launch_fn(bt.split2[pi][logL][kind == K_INVDFT], (unsigned)ntr, (unsigned)bt.split2_bt[pi][logL], args,
+                     bt.split2_lds[pi][logL], st);
+}
+
 static int c2c_axis(const Workspace& ws, Prec prec, int kind, int logL, const void* in, long long in_e,
                     void* out, long long out_e, const Levels& lv, hipStream_t st) {
     const int maxlog = device().maxlog[prec];
+    // radix-2-split kernel: always one past the tile limit (split2 >= 1),
+    // at the limit with split2 >= 2, one below it with split2 >= 3
+    const int sp2 = tune().split2 >= 0 ? tune().split2 : (prec == PREC_F64 ? 2 : 3);
+    if ((sp2 >= 1 && logL == maxlog + 1) || (sp2 >= 2 && logL == maxlog) || (sp2 >= 3 && logL == maxlog - 1)) {
+        const int rc = launch_split2(ws, prec, kind, logL, in, in_e, out, out_e, lv, st);
+        if (rc <= 0) return rc;
+    }
     if (logL <= maxlog)
         return launch_tile(prec, kind, logL, ws.tw, ws.tw64, in, in_e, out, out_e, lv, -1, 0, st, ws.split16 != 0);
-    // one past the tile limit, contiguous: single pass (split2.hpp)
-    {
-        const BigTable& bt = bigtable();
-        const int pi = prec == PREC_F64 ? 0 : 1;
-        if (tune().split2 && in_e == 1 && out_e == 1 && bt.split2[pi][0] && bt.split2_log[pi] == logL &&
-            bt.split2_lds[pi] <= (size_t)device().lds_max) {
-            TileArgs A;
-            std::memset(&A, 0, sizeof(A));
-            A.in = in;
-            A.out = out;
-            A.tw = ws.tw;
-            A.tw64 = (const double2*)ws.tw64;
-            A.in_e = 1;
-            A.out_e = 1;
-            fill_levels(A, lv);
-            const unsigned long long ntr = lv.count();
-            if (ntr == 0) return 0;
-            if (ntr > 0x7fffffffull) {
-                set_error("batch too large");
-                return -1;
-            }
-            A.ntrans = (unsigned)ntr;
-            A.ntiles = (unsigned)ntr;
-            A.C = 1;
-            A.xcd = 1;
-            void* args[] = {&A};
-            return launch_fn(bt.split2[pi][kind == K_INVDFT], (unsigned)ntr, (unsigned)bt.split2_bt[pi], args,
-                             bt.split2_lds[pi], st);
-        }
-    }
     if (in_e != out_e) {
         set_error("internal: two-step pass needs equal strides");
         return -1;

@@ -87,10 +87,14 @@ struct Tune {
     int cfb = 128;      // bytes per row segment of a column (c-fast) tile
     int wave = 1;       // wavefront-local exchanges for row tiles of <= 64 lanes per transform
     int xcd = 1;        // XCD-aware tile order for column tiles
+    int cfcore = -1;    // column tiles: 1 = c-fast core, 0 = row core + LDS-staged
+                        // column I/O, -1 = by kind (complex 1, real 0)
     int stage = 0;      // force LDS-staged complex I/O
     int chunk_mb = 64;  // two-step chunk size for entry batches
     int split = 0;      // two-step split of a long axis: log2 N1 = log2(N)/2 + split
-    int split2 = 1;     // single-pass radix-2-split kernel for the first length past one tile
+    int split2 = -1;    // radix-2-split single-pass kernel: 1 = for the first length past
+                        // one tile, 2 = also at the tile limit, 3 = also one below, 0 = off,
+                        // -1 = by precision (FP32 3, FP64 2: the measured best)
     int fused = 1;      // fused two-step kernel for batches of long 1D entries
     int fused_wpc = 2;  // work-groups per CU of the fused kernel
     int fused_alt = 0;  // use the alternative (smaller) fused work-groups
@@ -103,8 +107,8 @@ struct Tune {
 };
 // the schedule in force on this thread
 const Tune& tune();
-// applies "key=value,..." (the Tune fields: tpw, cfb, wave, xcd, stage,
-// chunk, split, split2, fused, wpc, falt) to t; false on an unknown key
+// applies "key=value,..." (the Tune fields: tpw, cfb, wave, xcd, cfcore,
+// stage, chunk, split, split2, fused, wpc, falt) to t; false on an unknown key
 bool parse_tune(const char* spec, Tune& t);
 // the process-wide schedule (defaults + MINFFT_TUNE)
 const Tune& base_tune();

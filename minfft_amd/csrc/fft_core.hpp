@@ -178,6 +178,13 @@ __device__ __forceinline__ void tile_sync(bool wave) {
     }
 }
 
+// Stockham pass twiddles: read every W^{rk} from the table (false) or only
+// W^{k}, W^{2k}, W^{4k}, W^{8k} and form the rest as products (true)
+#ifndef MINFFT_TW_POW2
+#define MINFFT_TW_POW2 1
+#endif
+constexpr bool kTwiddlePow2Loads = MINFFT_TW_POW2 != 0;
+
 template <typename T, int LOGL, int E, bool INV>
 struct Core {
     static constexpr int L = 1 << LOGL;
@@ -205,8 +212,24 @@ struct Core {
 #pragma unroll
                 for (int b = 0; b < NB; ++b) {
                     const int k = (tlo + b * TPT) & (Ns - 1);
+                    c2<T>* wb = w + b * (R - 1);  // wb[r-1] = W^{r k}
+                    if constexpr (kTwiddlePow2Loads && R >= 4) {
+                        // table reads for the powers-of-two multiples of k
+                        // only; the others as products of at most three of
+                        // them (a 64-lane read of W^{rk} spans up to r cache
+                        // lines, so this trades scattered reads for VALU)
 #pragma unroll
-                    for (int r = 1; r < R; ++r) w[b * (R - 1) + r - 1] = twl<T>(twr, (unsigned)(r * k));
+                        for (int p2 = 1; p2 < R; p2 *= 2) wb[p2 - 1] = twl<T>(twr, (unsigned)(p2 * k));
+#pragma unroll
+                        for (int r = 3; r < R; ++r) {
+                            if ((r & (r - 1)) == 0) continue;
+                            const int hi = 1 << (31 - __builtin_clz(r));  // highest power of two in r
+                            wb[r - 1] = cmul<T>(wb[hi - 1], wb[r - hi - 1]);
+                        }
+                    } else {
+#pragma unroll
+                        for (int r = 1; r < R; ++r) wb[r - 1] = twl<T>(twr, (unsigned)(r * k));
+                    }
                 }
             }
         }

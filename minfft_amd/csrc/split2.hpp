@@ -1,7 +1,9 @@
 // split2.hpp -- single-pass complex FFT of length L = 2H, one work-group
-// per transform, for the first length beyond one work-group's LDS (FP32
-// L = 2^15, FP64 L = 2^14): one HBM read and one HBM write per point
-// instead of the two of the two-step decomposition.
+// per transform, whose LDS holds only H points.  For the first length beyond
+// one work-group's LDS (FP32 L = 2^15, FP64 L = 2^14) it gives one HBM read
+// and one HBM write per point instead of the two of the two-step
+// decomposition; at the LDS limit itself (and one below) it halves the LDS
+// of a work-group, so that two share a CU and overlap memory and compute.
 //
 // Radix-2 decimation in frequency:
 //     a[n] = x[n] + x[n+H],   b[n] = (x[n] - x[n+H]) * W_L^n    (n < H)
@@ -17,8 +19,10 @@
 
 namespace mfx {
 
-template <typename T, int LOGH, int E, bool INV>
-__global__ void __launch_bounds__((1 << LOGH) / E) split2_kernel(TileArgs A) {
+// WPE: minimum waves per SIMD (register budget): 4 lets two 512-lane
+// work-groups share a CU when their LDS fits twice
+template <typename T, int LOGH, int E, bool INV, int WPE>
+__global__ void __launch_bounds__((1 << LOGH) / E, WPE) split2_kernel(TileArgs A) {
     using C2 = c2<T>;
     constexpr int H = 1 << LOGH, L = 2 * H, TPT = H / E;
     extern __shared__ __align__(16) unsigned char smem_raw[];

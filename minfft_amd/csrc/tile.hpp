@@ -42,8 +42,8 @@ struct TileArgs {
     int fs_logN;                       // 0: none; else multiply by W_{2^fs_logN}^{n2*k}
     int sc;                            // LDS slot length (complex points)
     int xcd;                           // number of XCDs for the tile swizzle (1 = off)
-    int wave;                          // 1: row mappings only, so a transform's
-                                       // lanes share one wavefront when TPT <= 64
+    int wave;                          // 1: row-mapped core, so a transform's lanes
+                                       // share one wavefront when TPT <= 64
 };
 
 struct TransformRef {
@@ -455,8 +455,11 @@ __global__ void __launch_bounds__(max_threads<T>()) tile_kernel(TileArgs A) {
     const int tl = A.cfast ? tl_cf : tl_row;
     const TransformRef me = locate(A, tile * (unsigned)C + (unsigned)c);
     C2* slot = lds + (long long)c * A.sc;
-    // every exchange stays inside a wavefront: no work-group barriers
+    // the core's exchanges stay inside a wavefront: no work-group barriers;
+    // staging with the column mapping still needs them
     const bool wv = (TPT <= 64) && A.wave;
+    const bool wv_in = wv && A.in_mode != IO_STAGE_CF;
+    const bool wv_out = wv && A.out_mode != IO_STAGE_CF;
 
     C2 v[E];
 
@@ -489,7 +492,7 @@ __global__ void __launch_bounds__(max_threads<T>()) tile_kernel(TileArgs A) {
         const TransformRef src = (sc_ == c) ? me : locate(A, tile * (unsigned)C + (unsigned)sc_);
         C2* sslot = lds + (long long)sc_ * A.sc;
         if (src.valid) stage_in<T, KIND, L, TPT>(A, src, sslot, stl);
-        tile_sync(wv);
+        tile_sync(wv_in);
         const T* rslot = reinterpret_cast<const T*>(slot);
         auto X = [&](int i) -> T { return rslot[ridx(i)]; };
         auto Z = [&](int i) -> C2 { return slot[lpad(i)]; };
@@ -548,7 +551,7 @@ __global__ void __launch_bounds__(max_threads<T>()) tile_kernel(TileArgs A) {
         }
     }
     Core<T, LOGL, E, INV>::write_owned(v, slot, tl);
-    tile_sync(wv);
+    tile_sync(wv_out);
     {
         const bool rowm = (A.out_mode != IO_STAGE_CF);
         const int sc_ = rowm ? c_row : c_cf;
