@@ -178,12 +178,36 @@ __device__ __forceinline__ void tile_sync(bool wave) {
     }
 }
 
+
 // Stockham pass twiddles: read every W^{rk} from the table (false) or only
 // W^{k}, W^{2k}, W^{4k}, W^{8k} and form the rest as products (true)
 #ifndef MINFFT_TW_POW2
 #define MINFFT_TW_POW2 1
 #endif
 constexpr bool kTwiddlePow2Loads = MINFFT_TW_POW2 != 0;
+
+// The R-1 twiddles W^{r k} (r = 1..R-1) of one butterfly from the table
+// twr[m] = W^m.  With kTwiddlePow2Loads, only the powers-of-two multiples of
+// k are read and the others formed as products of at most three of them: a
+// 64-lane read of W^{rk} spans up to r cache lines, so this trades scattered
+// reads (bound by the texture path) for a few VALU operations, at a cost of
+// a few ulp in the product twiddles.
+template <typename T, int R>
+__device__ __forceinline__ void butterfly_twiddles(c2<T>* wb, const c2<T>* __restrict__ twr, int k) {
+    if constexpr (kTwiddlePow2Loads && R >= 4) {
+#pragma unroll
+        for (int p2 = 1; p2 < R; p2 *= 2) wb[p2 - 1] = twl<T>(twr, (unsigned)(p2 * k));
+#pragma unroll
+        for (int r = 3; r < R; ++r) {
+            if ((r & (r - 1)) == 0) continue;
+            const int hi = 1 << (31 - __builtin_clz(r));  // highest power of two in r
+            wb[r - 1] = cmul<T>(wb[hi - 1], wb[r - hi - 1]);
+        }
+    } else {
+#pragma unroll
+        for (int r = 1; r < R; ++r) wb[r - 1] = twl<T>(twr, (unsigned)(r * k));
+    }
+}
 
 template <typename T, int LOGL, int E, bool INV>
 struct Core {
@@ -212,24 +236,7 @@ struct Core {
 #pragma unroll
                 for (int b = 0; b < NB; ++b) {
                     const int k = (tlo + b * TPT) & (Ns - 1);
-                    c2<T>* wb = w + b * (R - 1);  // wb[r-1] = W^{r k}
-                    if constexpr (kTwiddlePow2Loads && R >= 4) {
-                        // table reads for the powers-of-two multiples of k
-                        // only; the others as products of at most three of
-                        // them (a 64-lane read of W^{rk} spans up to r cache
-                        // lines, so this trades scattered reads for VALU)
-#pragma unroll
-                        for (int p2 = 1; p2 < R; p2 *= 2) wb[p2 - 1] = twl<T>(twr, (unsigned)(p2 * k));
-#pragma unroll
-                        for (int r = 3; r < R; ++r) {
-                            if ((r & (r - 1)) == 0) continue;
-                            const int hi = 1 << (31 - __builtin_clz(r));  // highest power of two in r
-                            wb[r - 1] = cmul<T>(wb[hi - 1], wb[r - hi - 1]);
-                        }
-                    } else {
-#pragma unroll
-                        for (int r = 1; r < R; ++r) wb[r - 1] = twl<T>(twr, (unsigned)(r * k));
-                    }
+                    butterfly_twiddles<T, R>(w + b * (R - 1), twr, k);
                 }
             }
         }
@@ -334,12 +341,10 @@ __device__ __forceinline__ void stockham_pass(c2<T> (&v)[E], c2<T>* slot, int tl
 #pragma unroll
         for (int r = 0; r < R; ++r) a[r] = v[b + r * NB];
         if constexpr (Ns > 1) {
-            const c2<T>* twr = tw + Ns * R;
+            c2<T> wv[R - 1];
+            butterfly_twiddles<T, R>(wv, tw + Ns * R, k);
 #pragma unroll
-            for (int r = 1; r < R; ++r) {
-                const c2<T> w = twl<T>(twr, (unsigned)(r * k));
-                a[r] = INV ? cmulc<T>(a[r], w) : cmul<T>(a[r], w);
-            }
+            for (int r = 1; r < R; ++r) a[r] = INV ? cmulc<T>(a[r], wv[r - 1]) : cmul<T>(a[r], wv[r - 1]);
         }
         dft<R, INV, T>(a);
         if constexpr (LAST) {
