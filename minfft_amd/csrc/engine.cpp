@@ -22,6 +22,7 @@
 #include "ktable.hpp"
 #include "tile.hpp"
 #include "big.hpp"
+#include "autotune.hpp"
 #include "coop.hpp"
 
 namespace mfx {
@@ -325,6 +326,10 @@ static int launch_tile(Prec prec, int kind, int logL, const void* tw, const void
         C = std::max(1, std::min(tn.tpw, maxT) / TPT);
     }
     if ((unsigned long long)C > ntr) C = (int)ntr;
+    // small batches: fewer transforms per tile until every CU gets two
+    // work-groups (wider row segments are worth less than idle CUs), down to
+    // one wavefront per work-group
+    while (C > 1 && (C / 2) * TPT >= 64 && (ntr + C - 1) / C < 2ull * (unsigned long long)D.ncu) C >>= 1;
     if (split16) C = 16;  // the MFMA tile is laid out for 16 transforms, 512 threads
     size_t budget = (size_t)D.lds_max;
     while (C > 1 && (size_t)C * sc * cs > budget) C >>= 1;
@@ -763,6 +768,25 @@ Plan* make_plan(Family fam, Prec prec, int d, const int* Ns) {
         if (hipMalloc(&p->tbuf, p->tbuf_bytes) != hipSuccess) return fail("hipMalloc scratch failed");
     }
     if (hipStreamSynchronize(p->stream) != hipSuccess) return fail("twiddle generation failed");
+    // launch schedule: timed on the device for large plans, with the
+    // family's forward kind standing for all of its kinds
+    p->sched = base_tune();
+    {
+        static const int rep[4] = {K_DFT, K_REALDFT, K_DCT2, K_DCT4};
+        TuneRequest r;
+        r.prec = prec;
+        r.kind = rep[fam];
+        r.d = std::min(d, 3);
+        for (int k = 0; k < r.d; ++k) r.n[k] = Ns[k];
+        r.count = 1;
+        r.tbuf = p->tbuf;
+        if (d <= 3) {
+            const TuneResult t = autotune_group(p->ws, r, p->stream);
+            p->sched = t.sched;
+            p->tuned = t.tuned;
+        }
+        clear_error();
+    }
     return p;
 }
 
@@ -882,6 +906,7 @@ int enqueue(Plan& p, int kind, const void* in, void* out, hipStream_t st) {
     if (kind == K_REALDFT && p.d > 1 && overlap(in, in_bytes(p, kind), out, out_bytes(p, kind)))
         tbuf = p.tbuf;
     if (kind == K_INVREALDFT && p.d > 1) tbuf = p.tbuf;
+    TuneScope scope(&p.sched);
     return run_entries(p.ws, p.prec, kind, p.d, p.Ns.data(), in, out, 1, nullptr, nullptr, nullptr, tbuf,
                        nullptr, st);
 }

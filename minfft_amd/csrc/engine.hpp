@@ -61,23 +61,6 @@ struct Workspace {
     int split16 = 0;           // FP32 data, radix-16 stages as split-FP16 MFMA products
 };
 
-struct Plan {
-    unsigned magic = 0x6d667870u;
-    Family fam = FAM_DFT;
-    Prec prec = PREC_F64;
-    int d = 0;
-    std::vector<int> Ns;
-    long long P = 0;          // product of dims
-    int dev = -1;
-    Workspace ws;
-    void* tbuf = nullptr;     // full-array scratch ("t" of the reference aux)
-    size_t tbuf_bytes = 0;
-    void* stage = nullptr;    // staging buffer for host pointers (lazy)
-    size_t stage_bytes = 0;
-    hipStream_t stream = nullptr;
-    bool own_stream = true;
-};
-
 // Launch schedule of the passes.  The defaults are the measured best for
 // large batches on MI355X; MINFFT_TUNE="key=value,..." overrides them for a
 // process, and batch plans pick one per group with the autotuner
@@ -105,6 +88,25 @@ struct Tune {
     int coop_la = 3;    // queue lookahead (rounds)
     int coop_s = 4;     // scratch slots per XCD (> lookahead)
 };
+struct Plan {
+    unsigned magic = 0x6d667870u;
+    Family fam = FAM_DFT;
+    Prec prec = PREC_F64;
+    int d = 0;
+    std::vector<int> Ns;
+    long long P = 0;          // product of dims
+    int dev = -1;
+    Workspace ws;
+    void* tbuf = nullptr;     // full-array scratch ("t" of the reference aux)
+    size_t tbuf_bytes = 0;
+    void* stage = nullptr;    // staging buffer for host pointers (lazy)
+    size_t stage_bytes = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = true;
+    Tune sched;               // launch schedule (autotuned for large plans)
+    bool tuned = false;
+};
+
 // the schedule in force on this thread
 const Tune& tune();
 // applies "key=value,..." (the Tune fields: tpw, cfb, wave, xcd, cfcore,
