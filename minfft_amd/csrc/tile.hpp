@@ -88,6 +88,18 @@ __device__ __forceinline__ unsigned tile_id(const TileArgs& A) {
     return (g % X) * per + g / X;
 }
 
+// Items a + s*(stl + k*TPT) (s = +-1) of a strided row, k = 0, 1, ...: one
+// 64-bit base per family of positions and a uniform step, instead of a
+// 64-bit multiply per item.
+template <typename P>
+struct Walk {
+    P* p0;
+    long long step;
+    __device__ __forceinline__ Walk(P* base, int a, int s, int stl, int TPT, long long es)
+        : p0(base + (long long)(a + s * stl) * es), step((long long)s * TPT * es) {}
+    __device__ __forceinline__ P& operator[](int k) const { return p0[k * step]; }
+};
+
 // 16-byte vectors of reals, for contiguous real rows
 template <typename T> struct Vec16;
 template <> struct Vec16<float> { using V = float4; static constexpr int n = 4; };
@@ -108,9 +120,10 @@ __device__ __forceinline__ void stage_in(const TileArgs& A, const TransformRef& 
         const C2* __restrict__ in = reinterpret_cast<const C2*>(A.in) + src.ib;
         constexpr int K = L / TPT;
         const long long es = A.in_e;
+        const Walk<const C2> wi(in, 0, 1, stl, TPT, es);
         C2 r[K];
 #pragma unroll
-        for (int k = 0; k < K; ++k) r[k] = in[(long long)(stl + k * TPT) * es];
+        for (int k = 0; k < K; ++k) r[k] = wi[k];
 #pragma unroll
         for (int k = 0; k < K; ++k) sslot[lpad(stl + k * TPT)] = r[k];
         if constexpr (kind_nin(KIND, L) > L) {  // inverse real DFT: item L
@@ -145,10 +158,10 @@ __device__ __forceinline__ void stage_in(const TileArgs& A, const TransformRef& 
             }
         }
         constexpr int K = NR / TPT;
-        const long long es = A.in_e;
+        const Walk<const T> wi(in, 0, 1, stl, TPT, A.in_e);
         T r[K];
 #pragma unroll
-        for (int k = 0; k < K; ++k) r[k] = in[(long long)(stl + k * TPT) * es];
+        for (int k = 0; k < K; ++k) r[k] = wi[k];
 #pragma unroll
         for (int k = 0; k < K; ++k) rs[ridx(stl + k * TPT)] = r[k];
     }
@@ -165,11 +178,9 @@ __device__ __forceinline__ void stage_out(const TileArgs& A, const TransformRef&
         C2* __restrict__ out = reinterpret_cast<C2*>(A.out) + dst.ob;
         constexpr int K = L / TPT;
         const long long es = A.out_e;
+        const Walk<C2> wo(out, 0, 1, stl, TPT, es);
 #pragma unroll
-        for (int k = 0; k < K; ++k) {
-            const int i = stl + k * TPT;
-            out[(long long)i * es] = post_item<T, KIND>(i, L, tw, Tf);
-        }
+        for (int k = 0; k < K; ++k) wo[k] = post_item<T, KIND>(stl + k * TPT, L, tw, Tf);
         if constexpr (kind_nout(KIND, L) > L) {  // real DFT: item L
             if (stl == 0) out[(long long)L * es] = post_item<T, KIND>(L, L, tw, Tf);
         }
@@ -196,12 +207,9 @@ __device__ __forceinline__ void stage_out(const TileArgs& A, const TransformRef&
             }
         }
         constexpr int K = NR / TPT;
-        const long long es = A.out_e;
+        const Walk<T> wo(out, 0, 1, stl, TPT, A.out_e);
 #pragma unroll
-        for (int k = 0; k < K; ++k) {
-            const int i = stl + k * TPT;
-            out[(long long)i * es] = post_item<T, KIND>(i, L, tw, Tf).x;
-        }
+        for (int k = 0; k < K; ++k) wo[k] = post_item<T, KIND>(stl + k * TPT, L, tw, Tf).x;
     }
 }
 
@@ -231,22 +239,51 @@ __device__ __forceinline__ void pre_owned(c2<T>* v, const c2<T>* slot, int tl, c
     if constexpr (KIND == K_DCT2 || KIND == K_DST2) {
         // t[n] = x[2n], t[L+n] = s*x[N-1-2n]; point p = (t[2p], t[2p+1])
         constexpr T sg = (KIND == K_DST2) ? T(-1) : T(1);
+        if constexpr (TPT % 8 == 0) {
+            // x[4p], x[4p+2] are the .x of complex positions 2p, 2p+1;
+            // x[N-1-4q], x[N-3-4q] the .y of L-1-2q, L-2-2q: padded
+            // positions linear in m (2*TPT is a multiple of 16)
+            constexpr int S = 2 * TPT + (2 * TPT) / 16;
+            const C2* lo = slot + lpad(2 * tl);
+            const C2* hi = slot + lpad(L - 1 - 2 * tl);
 #pragma unroll
-        for (int m = 0; m < E; ++m) {
-            const int p = tl + m * TPT;
-            if (m < E / 2) {
-                v[m] = mk<T>(X(4 * p), X(4 * p + 2));
-            } else {
-                const int q = p - H;
-                v[m] = mk<T>(sg * X(N - 1 - 4 * q), sg * X(N - 3 - 4 * q));
+            for (int m = 0; m < E; ++m) {
+                if (m < E / 2) {
+                    v[m] = mk<T>(lo[m * S].x, lo[m * S + 1].x);
+                } else {
+                    const int mq = m - E / 2;
+                    v[m] = mk<T>(sg * hi[-mq * S].y, sg * hi[-mq * S - 1].y);
+                }
+            }
+        } else {
+#pragma unroll
+            for (int m = 0; m < E; ++m) {
+                const int p = tl + m * TPT;
+                if (m < E / 2) {
+                    v[m] = mk<T>(X(4 * p), X(4 * p + 2));
+                } else {
+                    const int q = p - H;
+                    v[m] = mk<T>(sg * X(N - 1 - 4 * q), sg * X(N - 3 - 4 * q));
+                }
             }
         }
     } else if constexpr (KIND == K_DCT4 || KIND == K_DST4) {
+        // x[2p] is the .x of complex position p, x[N-1-2p] the .y of L-1-p
+        constexpr int S = TPT + TPT / 16;
+        const C2* lo = slot + lpad(tl);
+        const C2* hi = slot + lpad(L - 1 - tl);
 #pragma unroll
         for (int m = 0; m < E; ++m) {
             const int p = tl + m * TPT;
             const C2 e = twl<T>(tw, (unsigned)(2 * N + p));
-            const T xa = X(2 * p), xb = X(N - 1 - 2 * p);
+            T xa, xb;
+            if constexpr (TPT % 16 == 0) {
+                xa = lo[m * S].x;
+                xb = hi[-m * S].y;
+            } else {
+                xa = X(2 * p);
+                xb = X(N - 1 - 2 * p);
+            }
             if constexpr (KIND == K_DCT4)
                 v[m] = mk<T>(e.x * xa - e.y * xb, e.x * xb + e.y * xa);
             else
@@ -256,14 +293,24 @@ __device__ __forceinline__ void pre_owned(c2<T>* v, const c2<T>* slot, int tl, c
         // z[0..L] sits in the slot (the inverse real DFT's input; for
         // DCT-3/DST-3 put there by t3_to_z)
         auto zg = [&](int k) -> C2 { return slot[lpad(k)]; };
+        constexpr int S = TPT + TPT / 16;
+        const C2* zp = slot + lpad(tl);      // z[p] at m*S (TPT a multiple of 16)
+        const C2* zq = slot + lpad(L - tl);  // z[L-p] at -m*S
 #pragma unroll
         for (int m = 0; m < E; ++m) {
             const int p = tl + m * TPT;
             const bool lower = m < E / 2;  // p < L/2
-            // in [1, L/2] (point 0, whose result is replaced below, is
-            // clamped to 1 so that every read stays inside the slot)
-            const int mm = max(lower ? p : L - p, 1);
-            const C2 a = zg(mm), b = zg(L - mm);
+            // mm = min(p, L-p) in [0, L/2]: 0 only for point 0, whose result
+            // is replaced below (its reads, z[0] and z[L], are in the slot)
+            const int mm = lower ? p : L - p;
+            C2 a, b;
+            if constexpr (TPT % 16 == 0) {
+                a = lower ? zp[m * S] : zq[-m * S];
+                b = lower ? zq[-m * S] : zp[m * S];
+            } else {
+                a = zg(mm);
+                b = zg(L - mm);
+            }
             const C2 e = twl<T>(tw, (unsigned)(2 * L + mm));
             const T ur = a.x + b.x, ui = a.y - b.y;
             const T ttr = a.x - b.x, tti = a.y + b.y;
@@ -332,14 +379,15 @@ __device__ __forceinline__ void t3_to_z(c2<T>* slot, int tl, const c2<T>* __rest
     tile_sync(wave);
 }
 
-// two real outputs (2j, 2j+1) of a row; one 2-wide store when contiguous
+// two real outputs (2j, 2j+1) of a row, p = &item 2j; one 2-wide store when
+// contiguous
 template <typename T>
-__device__ __forceinline__ void store_pair(T* __restrict__ out, long long es, bool vec, int j, T a, T b) {
+__device__ __forceinline__ void store_pair(T* __restrict__ p, long long es, bool vec, T a, T b) {
     if (vec) {
-        reinterpret_cast<c2<T>*>(out)[j] = mk<T>(a, b);
+        *reinterpret_cast<c2<T>*>(p) = mk<T>(a, b);
     } else {
-        out[(long long)(2 * j) * es] = a;
-        out[(long long)(2 * j + 1) * es] = b;
+        p[0] = a;
+        p[es] = b;
     }
 }
 
@@ -360,14 +408,24 @@ __device__ __forceinline__ void post_real(const TileArgs& A, const TransformRef&
         auto im_pos = [&](int n) -> int { return DS ? n - 1 : N - n; };
         C2* __restrict__ oc = reinterpret_cast<C2*>(A.out) + dst.ob;
         T* __restrict__ orl = reinterpret_cast<T*>(A.out) + dst.ob;
-        auto emit = [&](int n, C2 z) {
-            if constexpr (KIND == K_REALDFT) {
-                oc[(long long)n * es] = z;
-            } else {
-                const C2 c = twl<T>(tw, (unsigned)(4 * N + n));
-                orl[(long long)re_pos(n) * es] = 2 * (z.x * c.x - z.y * c.y);
-                orl[(long long)im_pos(n) * es] = -2 * (z.x * c.y + z.y * c.x);
-            }
+        // output items of group m as walks over k (m = stl + k*TPT):
+        //   real DFT: Z[m] at m, Z[L-m] at L-m;
+        //   DCT-2: re(m) = m, im(m) = N-m, re(L-m) = L-m, im(L-m) = L+m;
+        //   DST-2: re(m) = N-1-m, im(m) = m-1, re(L-m) = L-1+m, im(L-m) = L-1-m.
+        const Walk<C2> zm(oc, 0, 1, stl, TPT, es), zl(oc, L, -1, stl, TPT, es);
+        const Walk<T> rm(orl, DS ? N - 1 : 0, DS ? -1 : 1, stl, TPT, es);
+        const Walk<T> im_(orl, DS ? -1 : N, DS ? 1 : -1, stl, TPT, es);
+        const Walk<T> rl(orl, DS ? L - 1 : L, DS ? 1 : -1, stl, TPT, es);
+        const Walk<T> il(orl, DS ? L - 1 : L, DS ? -1 : 1, stl, TPT, es);
+        // T[m] at tm[k*S], T[L-m] at tq[-k*S] (TPT a multiple of 16)
+        constexpr int S = TPT + TPT / 16;
+        const C2* tm = sslot + lpad(stl);
+        const C2* tq = sslot + lpad(L - stl);
+        // Re(Z c) and -Im(Z c) with c = W_{4N}^n
+        auto rot = [&](C2 z, int n, T& re, T& im) {
+            const C2 c = twl<T>(tw, (unsigned)(4 * N + n));
+            re = 2 * (z.x * c.x - z.y * c.y);
+            im = -2 * (z.x * c.y + z.y * c.x);
         };
 #pragma unroll
         for (int k = 0; k < E / 2; ++k) {
@@ -378,41 +436,75 @@ __device__ __forceinline__ void post_real(const TileArgs& A, const TransformRef&
                 if constexpr (KIND == K_REALDFT) {
                     oc[0] = mk<T>(u.x + u.y, (T)0);
                     oc[(long long)L * es] = mk<T>(u.x - u.y, (T)0);
+                    oc[(long long)H * es] = zh;
                 } else {
                     orl[(long long)re_pos(0) * es] = 2 * (u.x + u.y);
                     orl[(long long)re_pos(L) * es] = sqrt2_v<T>() * (u.x - u.y);
+                    T re, im;
+                    rot(zh, H, re, im);
+                    orl[(long long)re_pos(H) * es] = re;
+                    orl[(long long)im_pos(H) * es] = im;
                 }
-                emit(H, zh);
                 continue;
             }
-            const C2 a = Tf(m), b = Tf(L - m);
+            C2 a, b;
+            if constexpr (TPT % 16 == 0) {
+                a = tm[k * S];
+                b = tq[-k * S];
+            } else {
+                a = Tf(m);
+                b = Tf(L - m);
+            }
             const C2 e = twl<T>(tw, (unsigned)(2 * L + m));
             const T ur = (a.x + b.x) / 2, ui = (a.y - b.y) / 2;
             const T ttr = a.x - b.x, tti = a.y + b.y;
             const T vr = (ttr * e.y + tti * e.x) / 2;  // (ttr + i tti) * e / i / 2
             const T vi = (-ttr * e.x + tti * e.y) / 2;
-            emit(m, mk<T>(ur + vr, ui + vi));
-            emit(L - m, mk<T>(ur - vr, -ui + vi));
+            const C2 z0 = mk<T>(ur + vr, ui + vi), z1 = mk<T>(ur - vr, -ui + vi);  // Z[m], Z[L-m]
+            if constexpr (KIND == K_REALDFT) {
+                zm[k] = z0;
+                zl[k] = z1;
+            } else {
+                T re, im;
+                rot(z0, m, re, im);
+                rm[k] = re;
+                im_[k] = im;
+                rot(z1, L - m, re, im);
+                rl[k] = re;
+                il[k] = im;
+            }
         }
     } else {
         T* __restrict__ out = reinterpret_cast<T*>(A.out) + dst.ob;
         const bool vec = es == 1 && (reinterpret_cast<unsigned long long>(out) & (2 * sizeof(T) - 1)) == 0;
+        const Walk<T> w2(out, 0, 1, stl, TPT, 2 * es);  // item 2j
+        // linear padded positions (TPT a multiple of 32): T[j] at pj_[k*S],
+        // T[L-1-j] at pq_[-k*S]; T[j>>1] at ph_[k*S2], T[(N-1-j)>>1] at pr_[-k*S2]
+        constexpr int S = TPT + TPT / 16, S2 = TPT / 2 + TPT / 32;
+        const C2* pj_ = sslot + lpad(stl);
+        const C2* pq_ = sslot + lpad(L - 1 - stl);
+        const C2* ph_ = sslot + lpad(stl >> 1);
+        const C2* pr_ = sslot + lpad((N - 1 - stl) >> 1);
+        constexpr bool LIN = (TPT % 32 == 0);
 #pragma unroll
         for (int k = 0; k < E; ++k) {
             const int j = stl + k * TPT;  // 0 <= j < L
+            T* pj = &w2[k];
             if constexpr (KIND == K_INVREALDFT) {
-                const C2 t = Tf(j);
-                store_pair<T>(out, es, vec, j, t.x, t.y);
+                const C2 t = LIN ? pj_[k * S] : Tf(j);
+                store_pair<T>(pj, es, vec, t.x, t.y);
             } else if constexpr (KIND == K_DCT3 || KIND == K_DST3) {
                 // y[2j] = u[j], y[2j+1] = s*u[N-1-j], u = FFT output as reals
+                // (j and N-1-j keep the parity of stl and N-1-stl: TPT is even)
                 constexpr T sg = (KIND == K_DST3) ? T(-1) : T(1);
-                const C2 t0 = Tf(j >> 1);
                 const int r = N - 1 - j;
-                const C2 t1 = Tf(r >> 1);
-                store_pair<T>(out, es, vec, j, (j & 1) ? t0.y : t0.x, sg * ((r & 1) ? t1.y : t1.x));
+                const C2 t0 = LIN ? ph_[k * S2] : Tf(j >> 1);
+                const C2 t1 = LIN ? pr_[-k * S2] : Tf(r >> 1);
+                store_pair<T>(pj, es, vec, (j & 1) ? t0.y : t0.x, sg * ((r & 1) ? t1.y : t1.x));
             } else {  // DCT-4 / DST-4
                 const C2 e0 = twl<T>(tw, (unsigned)(8 * N + 4 * j + 1)), e1 = twl<T>(tw, (unsigned)(8 * N + 4 * j + 3));
-                const C2 t0 = Tf(j), t1 = Tf(L - 1 - j);
+                const C2 t0 = LIN ? pj_[k * S] : Tf(j);
+                const C2 t1 = LIN ? pq_[-k * S] : Tf(L - 1 - j);
                 T y0, y1;
                 if constexpr (KIND == K_DCT4) {
                     y0 = 2 * (e0.x * t0.x - e0.y * t0.y);
@@ -421,7 +513,7 @@ __device__ __forceinline__ void post_real(const TileArgs& A, const TransformRef&
                     y0 = 2 * (e0.x * t0.y + e0.y * t0.x);
                     y1 = 2 * (-e1.x * t1.y + e1.y * t1.x);
                 }
-                store_pair<T>(out, es, vec, j, y0, y1);
+                store_pair<T>(pj, es, vec, y0, y1);
             }
         }
     }
@@ -517,7 +609,7 @@ __global__ void __launch_bounds__(max_threads<T>()) tile_kernel(TileArgs A) {
     // two-step twiddle: W_N^{n2*p} on output point p = tl + m*TPT, computed
     // as W_N^{n2*tl} * (W_N^{n2*TPT})^m in FP64 (two table reads instead of E
     // scattered ones; the recurrence adds < 1e-15 relative error)
-    if (A.fs_logN > 0) {
+    if (CPLX && A.fs_logN > 0) {  // (two-step steps are complex passes)
         const unsigned n2 = me.at(A.fs_level);
         const unsigned long long Nfs = 1ull << A.fs_logN;
         const double2* twN = A.tw64 + Nfs;
