@@ -243,6 +243,12 @@ static void fill_levels(TileArgs& A, const Levels& lv, int* fs_level = nullptr) 
     if (fs_level && *fs_level >= 0) *fs_level = (tab && *fs_level == 0) ? 0 : off + *fs_level;
     A.in_off = tab ? lv.ioff : nullptr;
     A.out_off = tab ? lv.ooff : nullptr;
+    for (int k = 0; k < 4; ++k) {
+        const unsigned b = A.B[k];
+        A.lgB[k] = (b & (b - 1)) == 0 ? ilog2i((int)b) : -1;
+    }
+    A.C = 1;
+    A.lgC = 0;
 }
 
 // One launch of the fused tile kernel.
@@ -337,7 +343,10 @@ static int launch_tile(Prec prec, int kind, int logL, const void* tw, const void
         set_error("internal: LDS budget exceeded");
         return -1;
     }
+    // a power of two (the kernels split the lane index with shifts)
+    while (C & (C - 1)) C &= C - 1;
     A.C = C;
+    A.lgC = ilog2i(C);
     A.sc = sc;
     const unsigned long long tiles = (ntr + C - 1) / C;
     A.ntiles = (unsigned)tiles;

@@ -54,7 +54,7 @@ __global__ void __launch_bounds__(512) mfma512_kernel(TileArgs A) {
     const int C = A.C;  // 16 transforms per work-group (512 threads)
     const unsigned tile = tile_id(A);
     const int c_row = t / TPT, tl_row = t - c_row * TPT;
-    const int c_cf = t % C, tl_cf = t / C;
+    const int c_cf = t & (C - 1), tl_cf = t >> A.lgC;
     const int c = A.cfast ? c_cf : c_row;
     const int tl = A.cfast ? tl_cf : tl_row;
     const TransformRef me = locate(A, tile * (unsigned)C + (unsigned)c);

@@ -33,9 +33,11 @@ struct TileArgs {
     long long in_s[4], out_s[4];       // batch strides
     long long in_e, out_e;             // item strides
     unsigned B[4];                     // batch extents
+    int lgB[4];                        // log2 B[l] when B[l] is a power of two, else -1
     unsigned ntrans;                   // product of B
     unsigned ntiles;                   // grid size
-    int C;                             // transforms per tile
+    int C;                             // transforms per tile (a power of two)
+    int lgC;                           // log2 C
     int cfast;                         // core lane mapping: 1 = c-fast
     int in_mode, out_mode;             // IoMode
     int fs_level;                      // two-step twiddle: batch level holding n2
@@ -56,16 +58,26 @@ struct TransformRef {
     }
 };
 
+// n / B[l] and n % B[l]: shifts for power-of-two extents (the common case;
+// the branch is uniform), a division otherwise
+__device__ __forceinline__ unsigned divmod_level(const TileArgs& A, int l, unsigned n, unsigned& rem) {
+    if (A.lgB[l] >= 0) {
+        rem = n & (A.B[l] - 1u);
+        return n >> A.lgB[l];
+    }
+    const unsigned q = n / A.B[l];
+    rem = n - q * A.B[l];
+    return q;
+}
+
 __device__ __forceinline__ TransformRef locate(const TileArgs& A, unsigned b) {
     TransformRef r;
     r.valid = b < A.ntrans;
     if (!r.valid) b = 0;
-    unsigned i3 = b % A.B[3];
-    unsigned t = b / A.B[3];
-    unsigned i2 = t % A.B[2];
-    t /= A.B[2];
-    unsigned i1 = t % A.B[1];
-    unsigned i0 = t / A.B[1];
+    unsigned i3, i2, i1;
+    unsigned t = divmod_level(A, 3, b, i3);
+    t = divmod_level(A, 2, t, i2);
+    const unsigned i0 = divmod_level(A, 1, t, i1);
     r.i0 = i0; r.i1 = i1; r.i2 = i2; r.i3 = i3;
     long long ib = (A.in_off ? A.in_off[i0] : (long long)i0 * A.in_s[0]) +
                    (long long)i1 * A.in_s[1] + (long long)i2 * A.in_s[2] + (long long)i3 * A.in_s[3];
@@ -124,8 +136,12 @@ __device__ __forceinline__ void stage_in(const TileArgs& A, const TransformRef& 
         C2 r[K];
 #pragma unroll
         for (int k = 0; k < K; ++k) r[k] = wi[k];
+        C2* d0 = sslot + lpad(stl);
 #pragma unroll
-        for (int k = 0; k < K; ++k) sslot[lpad(stl + k * TPT)] = r[k];
+        for (int k = 0; k < K; ++k) {
+            if constexpr (TPT % 16 == 0) d0[k * (TPT + TPT / 16)] = r[k];
+            else sslot[lpad(stl + k * TPT)] = r[k];
+        }
         if constexpr (kind_nin(KIND, L) > L) {  // inverse real DFT: item L
             if (stl == 0) sslot[lpad(L)] = in[(long long)L * es];
         }
@@ -142,16 +158,19 @@ __device__ __forceinline__ void stage_in(const TileArgs& A, const TransformRef& 
                 V r[K];
 #pragma unroll
                 for (int k = 0; k < K; ++k) r[k] = reinterpret_cast<const V*>(in)[stl + k * TPT];
+                // first complex position p = (VN/2)(stl + k TPT); padded
+                // position linear in k when (VN/2) TPT is a multiple of 16
+                constexpr int PS = (VN / 2) * TPT;
+                C2* d0 = sslot + lpad((VN / 2) * stl);
 #pragma unroll
                 for (int k = 0; k < K; ++k) {
-                    const int p = (VN / 2) * (stl + k * TPT);  // first complex position
+                    C2* d = (PS % 16 == 0) ? d0 + k * (PS + PS / 16) : sslot + lpad((VN / 2) * (stl + k * TPT));
                     if constexpr (VN == 4) {
                         // positions p, p+1 (p even: never split by a pad gap)
-                        C2* d = sslot + lpad(p);
                         d[0] = mk<T>(r[k].x, r[k].y);
                         d[1] = mk<T>(r[k].z, r[k].w);
                     } else {
-                        sslot[lpad(p)] = mk<T>(r[k].x, r[k].y);
+                        d[0] = mk<T>(r[k].x, r[k].y);
                     }
                 }
                 return;
@@ -162,8 +181,14 @@ __device__ __forceinline__ void stage_in(const TileArgs& A, const TransformRef& 
         T r[K];
 #pragma unroll
         for (int k = 0; k < K; ++k) r[k] = wi[k];
+        // real item i = stl + k TPT sits at ridx(i); linear in k when TPT/2
+        // is a multiple of 16 (TPT even keeps the parity of i)
+        T* d0 = rs + ridx(stl);
 #pragma unroll
-        for (int k = 0; k < K; ++k) rs[ridx(stl + k * TPT)] = r[k];
+        for (int k = 0; k < K; ++k) {
+            if constexpr (TPT % 32 == 0) d0[k * (TPT + TPT / 16)] = r[k];
+            else rs[ridx(stl + k * TPT)] = r[k];
+        }
     }
 }
 
@@ -542,7 +567,7 @@ __global__ void __launch_bounds__(max_threads<T>()) tile_kernel(TileArgs A) {
     const unsigned tile = tile_id(A);
     // the two lane mappings
     const int c_row = t / TPT, tl_row = t - c_row * TPT;
-    const int c_cf = t % C, tl_cf = t / C;
+    const int c_cf = t & (C - 1), tl_cf = t >> A.lgC;
     const int c = A.cfast ? c_cf : c_row;
     const int tl = A.cfast ? tl_cf : tl_row;
     const TransformRef me = locate(A, tile * (unsigned)C + (unsigned)c);
