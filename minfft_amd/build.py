@@ -31,8 +31,8 @@ COMMON = [
     "-Wall", "-Wno-unused-function", "-Wno-unused-variable", "-Wno-unused-but-set-variable",
 ]
 
-HEADERS = ["fft_core.hpp", "kinds.hpp", "tile.hpp", "ktable.hpp", "engine.hpp", "big.hpp", "bigtable.hpp", "coop.hpp",
-           "mfma.hpp", "autotune.hpp", "split2.hpp"]
+HEADERS = ["fft_core.hpp", "kinds.hpp", "tile.hpp", "ktable.hpp", "engine.hpp", "big.hpp", "bigtable.hpp", "mfma.hpp",
+           "autotune.hpp", "split2.hpp"]
 
 
 def _newer(target, deps):

@@ -214,15 +214,6 @@ minfft_batch_plan* minfft_mkaux_batch(int count, const minfft_batch_entry* e, in
     });
     if (alloc_twiddles(p->ws, p->prec, twlog, p->stream)) return fail(nullptr);
     p->ws.split16 = (prec == MINFFT_SPLIT_FP16) ? 1 : 0;
-    {
-        // work-queue counters of the cooperative kernel (largest 1D group)
-        size_t mx = 0;
-        for (auto& g : p->groups)
-            if (g.d == 1) mx = std::max(mx, g.ioff.size());
-        p->ws.nctrs = 8 + 2 * mx + 1;
-        if (hipMalloc(&p->ws.ctrs, p->ws.nctrs * sizeof(unsigned)) != hipSuccess)
-            return fail("hipMalloc batch counters failed");
-    }
     if (fs_max) {
         if (hipMalloc(&p->ws.fbuf, fs_max) != hipSuccess) return fail("hipMalloc batch scratch failed");
         if (hipMalloc(&p->ws.gbuf, fs_max) != hipSuccess) return fail("hipMalloc batch scratch failed");

@@ -37,8 +37,6 @@ static void reg_split2(BigTable& t, int prec) {
 void register_big(BigTable& t) {
     t.mfma512[0] = reinterpret_cast<const void*>(&mfma512_kernel<false>);
     t.mfma512[1] = reinterpret_cast<const void*>(&mfma512_kernel<true>);
-    // the XCD-cooperative kernel (coop.hpp) is not instantiated: its path is off
-    t.coop_reset = nullptr;
     // <T, log2 H, min waves per SIMD>: the first length past one
     // work-group's LDS, and the two lengths below it (two work-groups per CU)
     reg_split2<float, 14, 4>(t, 1);

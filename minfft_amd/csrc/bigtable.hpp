@@ -14,12 +14,6 @@ struct BigTable {
     size_t alt_lds[2][17];
     int alt_bt[2][17];
     int alt_wpc[2][17];
-    // XCD-cooperative kernels (coop.hpp), two launch-shape variants
-    const void* coop[2][17][2][2];
-    size_t coop_lds[2][17][2];
-    int coop_bt[2][17][2];
-    int coop_wpc[2][17][2];
-    const void* coop_reset;
     // 512-point FP32 tiles with the split-FP16 MFMA radix-16 stage (mfma.hpp)
     const void* mfma512[2];
     // single-pass radix-2-split kernels (split2.hpp), by log2 L

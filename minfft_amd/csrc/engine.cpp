@@ -23,7 +23,6 @@
 #include "tile.hpp"
 #include "big.hpp"
 #include "autotune.hpp"
-#include "coop.hpp"
 
 namespace mfx {
 
@@ -151,8 +150,8 @@ bool is_device_pointer(const void* p) {
 
 // ------------------------------------------------------------ tile tuning
 // Process-wide schedule: defaults, then MINFFT_TUNE="key=value,..." (keys:
-// tpw, cfb, wave, xcd, stage, chunk, split, fused, wpc, falt, cv, cwpc, cla,
-// cs -- the fields of Tune in engine.hpp).
+// tpw, cfb, cfcore, wave, xcd, stage, chunk, split, split2, fused, wpc, falt
+// -- the fields of Tune in engine.hpp).
 static Tune g_tune;
 static std::once_flag g_tune_once;
 static thread_local const Tune* t_tune = nullptr;
@@ -186,13 +185,8 @@ bool parse_tune(const char* spec, Tune& t) {
         else if (k == "fused") t.fused = v;
         else if (k == "wpc" && v > 0) t.fused_wpc = v;
         else if (k == "falt") t.fused_alt = v;
-        else if (k == "cv") t.coop_v = v;
-        else if (k == "cwpc" && v > 0) t.coop_wpc = v;
-        else if (k == "cla" && v >= 0) t.coop_la = v;
-        else if (k == "cs" && v > 0) t.coop_s = v;
         else ok = false;
     }
-    t.coop = 0;  // never selectable
     return ok;
 }
 
@@ -485,44 +479,6 @@ static int c2c_axis(const Workspace& ws, Prec prec, int kind, int logL, const vo
         set_error("internal: offset table level clash");
         return -1;
     }
-    // Batches of contiguous 1D entries: the XCD-cooperative kernel keeps the
-    // intermediate in L2 (reset kernel + one launch).
-    {
-        const BigTable& bt = bigtable();
-        const Tune& tn = tune();
-        const int pi = prec == PREC_F64 ? 0 : 1;
-        const int lg = std::min(logL, 16);
-        const int v = tn.coop_v & 1;
-        const void* fn = (logL <= 16) ? bt.coop[pi][lg][v][kind == K_INVDFT] : nullptr;
-        const unsigned cnt = lv.n ? lv.B[0] : 1;
-        const int S = std::max(tn.coop_s, tn.coop_la + 1);
-        const size_t need = (size_t)kXcds * S * ((size_t)1 << logL) * (prec == PREC_F64 ? 16 : 8);
-        const size_t nct = (size_t)kXcds + 2 * (size_t)cnt + 1;
-        if (fn && tn.coop && lv.n == 1 && lv.ioff && lv.ooff && e == 1 && cnt >= (unsigned)kXcds &&
-            need <= ws.fs_bytes && ws.ctrs && nct <= ws.nctrs) {
-            unsigned n = (unsigned)nct;
-            unsigned* cptr = ws.ctrs;
-            void* rargs[] = {&cptr, &n};
-            int rc = launch_fn(bt.coop_reset, (n + 255) / 256, 256, rargs, 0, st);
-            if (rc) return rc;
-            CoopArgs C;
-            C.in = in;
-            C.out = out;
-            C.scratch = ws.fbuf;
-            C.tw = ws.tw;
-            C.tw64 = (const double2*)ws.tw64;
-            C.in_off = lv.ioff;
-            C.out_off = lv.ooff;
-            C.ctrs = ws.ctrs;
-            C.count = cnt;
-            C.slots = S;
-            C.lookahead = tn.coop_la;
-            const int wpc = std::max(1, std::min(tn.coop_wpc, bt.coop_wpc[pi][lg][v]));
-            const unsigned grid = (unsigned)(device().ncu * wpc);  // a multiple of the XCD count
-            void* args[] = {&C};
-            return launch_fn(fn, grid, (unsigned)bt.coop_bt[pi][lg][v], args, bt.coop_lds[pi][lg][v], st);
-        }
-    }
     // Batches of contiguous 1D entries: one fused launch, a work-group per
     // transform, intermediate in a per-work-group scratch slot.
     {
@@ -672,7 +628,6 @@ void free_workspace(Workspace& ws) {
     if (ws.tw64) (void)hipFree(ws.tw64);
     if (ws.fbuf) (void)hipFree(ws.fbuf);
     if (ws.gbuf) (void)hipFree(ws.gbuf);
-    if (ws.ctrs) (void)hipFree(ws.ctrs);
     ws = Workspace();
 }
 

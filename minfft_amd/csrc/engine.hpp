@@ -56,8 +56,6 @@ struct Workspace {
     void* fbuf = nullptr;      // two-step intermediate
     void* gbuf = nullptr;      // two-step complex staging for real kinds
     size_t fs_bytes = 0;
-    unsigned* ctrs = nullptr;  // work-queue counters of the cooperative kernel
-    size_t nctrs = 0;
     int split16 = 0;           // FP32 data, radix-16 stages as split-FP16 MFMA products
 };
 
@@ -81,12 +79,6 @@ struct Tune {
     int fused = 1;      // fused two-step kernel for batches of long 1D entries
     int fused_wpc = 2;  // work-groups per CU of the fused kernel
     int fused_alt = 0;  // use the alternative (smaller) fused work-groups
-    int coop = 0;       // XCD-cooperative kernel (coop.hpp): off and not selectable --
-                        // two GPU runs hung (see the header of coop.hpp)
-    int coop_v = 0;     // its launch-shape variant
-    int coop_wpc = 4;   // its work-groups per CU
-    int coop_la = 3;    // queue lookahead (rounds)
-    int coop_s = 4;     // scratch slots per XCD (> lookahead)
 };
 struct Plan {
     unsigned magic = 0x6d667870u;
