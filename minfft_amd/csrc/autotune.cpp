@@ -9,8 +9,8 @@
 // decomposition.  The tuner runs the
 // group's passes on a bounded subset of its entries (at most ~512 MiB of
 // input plus output, placed contiguously in scratch buffers) and walks the
-// knobs that apply to the group one at a time (coordinate descent), keeping
-// a value when it is at least 3 % faster.  Results are cached per (precision,
+// knobs that apply to the group one at a time (coordinate descent, up to
+// three rounds), keeping a value when it is at least 3 % faster.  Results are cached per (precision,
 // kind, shape, count bucket) for the process, and in the file named by
 // MINFFT_TUNE_CACHE if set.  MINFFT_AUTOTUNE=0, or a MINFFT_TUNE override,
 // turns the tuner off (the process-wide schedule is used).
@@ -53,7 +53,7 @@ std::string key_of(Prec prec, int kind, int d, const int* n, unsigned count) {
     return b;
 }
 
-// cache file line: key tpw cfb cfcore wave xcd split split2 fused wpc falt us_default us_best
+// cache file line: key tpw cfb cfcore wave xcd e32 split split2 fused wpc falt us_default us_best
 void load_cache() {
     if (g_loaded) return;
     g_loaded = true;
@@ -67,7 +67,7 @@ void load_cache() {
         TuneResult r;
         r.sched = base_tune();
         Tune& t = r.sched;
-        if (ss >> k >> t.tpw >> t.cfb >> t.cfcore >> t.wave >> t.xcd >> t.split >> t.split2 >> t.fused >>
+        if (ss >> k >> t.tpw >> t.cfb >> t.cfcore >> t.wave >> t.xcd >> t.e32 >> t.split >> t.split2 >> t.fused >>
             t.fused_wpc >> t.fused_alt >>
             r.us_default >> r.us_best) {
             r.tuned = true;
@@ -81,7 +81,8 @@ void save_entry(const std::string& k, const TuneResult& r) {
     if (!f) return;
     std::ofstream out(f, std::ios::app);
     const Tune& t = r.sched;
-    out << k << ' ' << t.tpw << ' ' << t.cfb << ' ' << t.cfcore << ' ' << t.wave << ' ' << t.xcd << ' ' << t.split
+    out << k << ' ' << t.tpw << ' ' << t.cfb << ' ' << t.cfcore << ' ' << t.wave << ' ' << t.xcd << ' ' << t.e32
+        << ' ' << t.split
         << ' ' << t.split2
         << ' ' << t.fused
         << ' ' << t.fused_wpc << ' ' << t.fused_alt << ' ' << r.us_default << ' ' << r.us_best << '\n';
@@ -146,6 +147,15 @@ TuneResult autotune_group(const Workspace& ws, const TuneRequest& r, hipStream_t
         knobs.push_back({&Tune::cfb, {128, 64, 256, 32}});
         knobs.push_back({&Tune::xcd, {1, 0}});
     }
+    {
+        // radix-32 tiles and radix-2-split halves: FP32 complex passes of
+        // 2^10 .. 2^(maxlog+1) points
+        bool mid = false;
+        for (int k = 0; k < r.d; ++k)
+            if (r.n[k] >= 1024 && r.n[k] <= (2 << maxlog)) mid = true;
+        if (mid && r.prec == PREC_F32 && (r.kind == K_DFT || r.kind == K_INVDFT))
+            knobs.push_back({&Tune::e32, {0, 1}});
+    }
     knobs.push_back({&Tune::tpw, {256, 128, 512}});
     if (short_row) knobs.push_back({&Tune::wave, {1, 0}});
 
@@ -203,17 +213,24 @@ TuneResult autotune_group(const Workspace& ws, const TuneRequest& r, hipStream_t
     float tbest = measure(best);
     const float tdef = tbest;
     if (std::isfinite(tbest)) {
-        for (const Knob& k : knobs) {
-            for (int v : k.values) {
-                if (best.*(k.field) == v) continue;
-                Tune c = best;
-                c.*(k.field) = v;
-                const float t = measure(c);
-                if (t < 0.97f * tbest) {
-                    best = c;
-                    tbest = t;
+        // coordinate descent, repeated while a round changes something
+        // (knobs interact: the radix-2-split level and the radix-32 choice)
+        for (int round = 0; round < 3; ++round) {
+            bool changed = false;
+            for (const Knob& k : knobs) {
+                for (int v : k.values) {
+                    if (best.*(k.field) == v) continue;
+                    Tune c = best;
+                    c.*(k.field) = v;
+                    const float t = measure(c);
+                    if (t < 0.97f * tbest) {
+                        best = c;
+                        tbest = t;
+                        changed = true;
+                    }
                 }
             }
+            if (!changed) break;
         }
         res.sched = best;
         res.tuned = true;
@@ -232,8 +249,8 @@ TuneResult autotune_group(const Workspace& ws, const TuneRequest& r, hipStream_t
 
 std::string describe_tune(const Tune& t) {
     char b[160];
-    std::snprintf(b, sizeof(b), "tpw=%d cfb=%d cfcore=%d wave=%d xcd=%d split=%d split2=%d fused=%d wpc=%d falt=%d",
-                  t.tpw, t.cfb, t.cfcore, t.wave, t.xcd, t.split, t.split2, t.fused, t.fused_wpc, t.fused_alt);
+    std::snprintf(b, sizeof(b), "tpw=%d cfb=%d cfcore=%d wave=%d xcd=%d e32=%d split=%d split2=%d fused=%d wpc=%d falt=%d",
+                  t.tpw, t.cfb, t.cfcore, t.wave, t.xcd, t.e32, t.split, t.split2, t.fused, t.fused_wpc, t.fused_alt);
     return b;
 }
 

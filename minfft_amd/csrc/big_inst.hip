@@ -26,11 +26,13 @@ static void reg_one(BigTable& t, int prec, int variant = 0) {
     t.wpc[prec][lg] = (4 * WPE * 64) / BT;  // work-groups per CU the registers allow
 }
 
-template <typename T, int LOGH, int WPE>
+// variant 0: 16 points per lane; variant 1 (E = 32): 32 points per lane
+template <typename T, int LOGH, int WPE, int E = 16>
 static void reg_split2(BigTable& t, int prec) {
-    t.split2[prec][LOGH + 1][0] = reinterpret_cast<const void*>(&split2_kernel<T, LOGH, 16, false, WPE>);
-    t.split2[prec][LOGH + 1][1] = reinterpret_cast<const void*>(&split2_kernel<T, LOGH, 16, true, WPE>);
-    t.split2_bt[prec][LOGH + 1] = (1 << LOGH) / 16;
+    constexpr int v = E == 32 ? 1 : 0;
+    t.split2[prec][LOGH + 1][v][0] = reinterpret_cast<const void*>(&split2_kernel<T, LOGH, E, false, WPE>);
+    t.split2[prec][LOGH + 1][v][1] = reinterpret_cast<const void*>(&split2_kernel<T, LOGH, E, true, WPE>);
+    t.split2_bt[prec][LOGH + 1][v] = (1 << LOGH) / E;
     t.split2_lds[prec][LOGH + 1] = split2_lds_bytes<T, LOGH>();
 }
 
@@ -45,6 +47,8 @@ void register_big(BigTable& t) {
     reg_split2<double, 13, 2>(t, 0);
     reg_split2<double, 12, 2>(t, 0);
     reg_split2<double, 11, 2>(t, 0);
+    reg_split2<float, 14, 2, 32>(t, 1);
+    reg_split2<float, 13, 2, 32>(t, 1);
     // <T, log2 N1, log2 N2, threads, min waves per SIMD>
     reg_one<float, 7, 7, 512, 2>(t, 1);
     reg_one<float, 8, 7, 512, 4>(t, 1);

@@ -16,9 +16,10 @@ struct BigTable {
     int alt_wpc[2][17];
     // 512-point FP32 tiles with the split-FP16 MFMA radix-16 stage (mfma.hpp)
     const void* mfma512[2];
-    // single-pass radix-2-split kernels (split2.hpp), by log2 L
-    const void* split2[2][17][2];  // [prec][log2 L][inverse]
-    int split2_bt[2][17];
+    // single-pass radix-2-split kernels (split2.hpp), by log2 L; variant 1
+    // runs the halves with 32 points per lane (radix-32 passes)
+    const void* split2[2][17][2][2];  // [prec][log2 L][variant][inverse]
+    int split2_bt[2][17][2];
     size_t split2_lds[2][17];
 };
 

@@ -150,8 +150,8 @@ bool is_device_pointer(const void* p) {
 
 // ------------------------------------------------------------ tile tuning
 // Process-wide schedule: defaults, then MINFFT_TUNE="key=value,..." (keys:
-// tpw, cfb, cfcore, wave, xcd, stage, chunk, split, split2, fused, wpc, falt
-// -- the fields of Tune in engine.hpp).
+// tpw, cfb, cfcore, wave, xcd, e32, stage, chunk, split, split2, fused, wpc,
+// falt -- the fields of Tune in engine.hpp).
 static Tune g_tune;
 static std::once_flag g_tune_once;
 static thread_local const Tune* t_tune = nullptr;
@@ -182,6 +182,7 @@ bool parse_tune(const char* spec, Tune& t) {
         else if (k == "split") t.split = v;
         else if (k == "split2") t.split2 = v;
         else if (k == "cfcore") t.cfcore = v;
+        else if (k == "e32") t.e32 = v;
         else if (k == "fused") t.fused = v;
         else if (k == "wpc" && v > 0) t.fused_wpc = v;
         else if (k == "falt") t.fused_alt = v;
@@ -257,8 +258,9 @@ static int launch_tile(Prec prec, int kind, int logL, const void* tw, const void
         return -1;
     }
     split16 = split16 && prec == PREC_F32 && logL == 9 && (kind == K_DFT || kind == K_INVDFT) && fs_logN == 0;
-    const void* fn = split16 ? bigtable().mfma512[kind == K_INVDFT] : kt.tile[kind][logL];
-    const int E = split16 ? 16 : kt.tile_E[logL];
+    const bool e32 = !split16 && tune().e32 && kt.tile32[kind][logL];
+    const void* fn = split16 ? bigtable().mfma512[kind == K_INVDFT] : (e32 ? kt.tile32[kind][logL] : kt.tile[kind][logL]);
+    const int E = split16 ? 16 : (e32 ? 32 : kt.tile_E[logL]);
     const int L = 1 << logL;
     const int TPT = L / E;
     TileArgs A;
@@ -314,7 +316,7 @@ static int launch_tile(Prec prec, int kind, int logL, const void* tw, const void
     const bool wants_cf = A.cfast || A.in_mode == IO_STAGE_CF || A.out_mode == IO_STAGE_CF;
     const size_t cs = prec == PREC_F64 ? 16 : 8;
     const int sc = slot_len(L);
-    const int maxT = prec == PREC_F64 ? 512 : 1024;  // max_threads<T>() of tile.hpp
+    const int maxT = (prec == PREC_F64 || e32) ? 512 : 1024;  // tile_threads<T, E>() of tile.hpp
     int C;
     if (wants_cf) {
         // several columns per tile so that each row segment of the strided
@@ -385,9 +387,10 @@ static int launch_split2(const Workspace& ws, Prec prec, int kind, int logL, con
                          void* out, long long out_e, const Levels& lv, hipStream_t st) {
     const BigTable& bt = bigtable();
     const int pi = prec == PREC_F64 ? 0 : 1;
-    if (in_e != 1 || out_e != 1 || logL > 16 || !bt.split2[pi][logL][0] ||
+    if (in_e != 1 || out_e != 1 || logL > 16 || !bt.split2[pi][logL][0][0] ||
         bt.split2_lds[pi][logL] > (size_t)device().lds_max)
         return 1;
+    const int v = (tune().e32 && bt.split2[pi][logL][1][0]) ? 1 : 0;
     TileArgs A;
     std::memset(&A, 0, sizeof(A));
     A.in = in;
@@ -408,8 +411,8 @@ static int launch_split2(const Workspace& ws, Prec prec, int kind, int logL, con
     A.C = 1;
     A.xcd = 1;
     void* args[] = {&A};
-    return launch_fn(bt.split2[pi][logL][kind == K_INVDFT], (unsigned)ntr, (unsigned)bt.split2_bt[pi][logL], args,
-                     bt.split2_lds[pi][logL], st);
+    return launch_fn(bt.split2[pi][logL][v][kind == K_INVDFT], (unsigned)ntr, (unsigned)bt.split2_bt[pi][logL][v],
+                     args, bt.split2_lds[pi][logL], st);
 }
 
 static int c2c_axis(const Workspace& ws, Prec prec, int kind, int logL, const void* in, long long in_e,

@@ -73,6 +73,7 @@ struct Tune {
     int stage = 0;      // force LDS-staged complex I/O
     int chunk_mb = 64;  // two-step chunk size for entry batches
     int split = 0;      // two-step split of a long axis: log2 N1 = log2(N)/2 + split
+    int e32 = 0;        // radix-32 tiles (32 points per lane) where instantiated
     int split2 = -1;    // radix-2-split single-pass kernel: 1 = for the first length past
                         // one tile, 2 = also at the tile limit, 3 = also one below, 0 = off,
                         // -1 = by precision (FP32 3, FP64 2: the measured best)
@@ -102,7 +103,8 @@ struct Plan {
 // the schedule in force on this thread
 const Tune& tune();
 // applies "key=value,..." (the Tune fields: tpw, cfb, wave, xcd, cfcore,
-// stage, chunk, split, split2, fused, wpc, falt) to t; false on an unknown key
+// e32, stage, chunk, split, split2, fused, wpc, falt) to t; false on an
+// unknown key
 bool parse_tune(const char* spec, Tune& t);
 // the process-wide schedule (defaults + MINFFT_TUNE)
 const Tune& base_tune();

@@ -302,10 +302,17 @@ struct Core {
                 for (int r = 0; r < R; ++r) v[b + r * NB] = a[r];
             } else {
                 const int base = (j - k) * R + k;
-                if constexpr (Ns == 1 || (Ns % 16) == 0) {
+                if constexpr (Ns == 1) {
+                    // base = R*j: positions base + r, r < R; for R <= 16 they
+                    // stay in one 16-point block, for R = 32 the block edge
+                    // falls at r = 16 -- both give lpad(base) + r + r/16
+                    c2<T>* sb = slot + lpad(base);
+#pragma unroll
+                    for (int r = 0; r < R; ++r) sb[r + (r >> 4)] = a[r];
+                } else if constexpr ((Ns % 16) == 0) {
                     // padded index is linear in r: one base, immediate offsets
                     c2<T>* sb = slot + lpad(base);
-                    constexpr int step = (Ns == 1) ? 1 : Ns + Ns / 16;
+                    constexpr int step = Ns + Ns / 16;
 #pragma unroll
                     for (int r = 0; r < R; ++r) sb[r * step] = a[r];
                 } else {

@@ -19,8 +19,9 @@
 
 namespace mfx {
 
-// WPE: minimum waves per SIMD (register budget): 4 lets two 512-lane
-// work-groups share a CU when their LDS fits twice
+// E: points per lane of the half-length FFTs (16, or 32 for radix-32
+// passes); WPE: minimum waves per SIMD (register budget): 4 lets two
+// 512-lane work-groups share a CU when their LDS fits twice
 template <typename T, int LOGH, int E, bool INV, int WPE>
 __global__ void __launch_bounds__((1 << LOGH) / E, WPE) split2_kernel(TileArgs A) {
     using C2 = c2<T>;

@@ -547,9 +547,11 @@ __device__ __forceinline__ void post_real(const TileArgs& A, const TransformRef&
 // Largest work-group of a tile kernel: FP64 radix-16 butterflies need more
 // than the 128 VGPRs a 1024-thread group leaves each lane.
 template <typename T> constexpr int max_threads() { return sizeof(T) == 8 ? 512 : 1024; }
+// radix-32 tiles (E = 32) hold twice the points per lane: at most 512 lanes
+template <typename T, int E> constexpr int tile_threads() { return E >= 32 ? 512 : max_threads<T>(); }
 
 template <typename T, int LOGL, int E, int KIND>
-__global__ void __launch_bounds__(max_threads<T>()) tile_kernel(TileArgs A) {
+__global__ void __launch_bounds__((tile_threads<T, E>())) tile_kernel(TileArgs A) {
     using C2 = c2<T>;
     constexpr int L = 1 << LOGL;
     constexpr int TPT = L / E;

@@ -13,7 +13,8 @@ pytestmark = pytest.mark.gpu
 # every knob value the tuner tries (autotune.cpp), alone and in the
 # combinations that change the long-axis path
 SPECS = ["", "tpw=128", "tpw=512", "cfb=64", "cfb=256", "xcd=0", "wave=0",
-         "cfcore=0", "cfcore=1", "cfcore=0,cfb=256", "cfcore=0,wave=0",
+         "cfcore=0", "cfcore=1", "cfcore=0,cfb=256", "cfcore=0,wave=0", "e32=1", "e32=1,split2=1",
+         "e32=1,split2=0,fused=0", "e32=1,cfcore=0",
          "fused=0", "falt=1", "wpc=1", "wpc=4", "split=-1", "split=1",
          "split2=0", "split2=0,split=-1", "split2=0,split=1", "split2=1", "split2=2", "split2=3",
          "fused=0,split=-1", "fused=0,split=1"]
@@ -73,6 +74,7 @@ def _cases(prec):
     fused_n, split_n = _long(prec)
     entries = []
     entries += [("dft", (64, 256))] * 24 + [("invdft", (128, 32, 16))] * 4
+    entries += [("dft", (8, 2048))] * 4 + [("invdft", (1024, 8))] * 4
     entries += [("dft", (fused_n,))] * 260 + [("invdft", (split_n,))] * 40
     entries += [("dct2", (128, 128))] * 8 + [("dst4", (64, 32))] * 8
     entries += [("dft", (split_n // 2,))] * 6 + [("invdft", (split_n // 4,))] * 6
