@@ -148,13 +148,15 @@ TuneResult autotune_group(const Workspace& ws, const TuneRequest& r, hipStream_t
         knobs.push_back({&Tune::xcd, {1, 0}});
     }
     {
-        // radix-32 tiles and radix-2-split halves: FP32 complex passes of
-        // 2^10 .. 2^(maxlog+1) points
+        // radix-32 tiles and radix-2-split halves (FP32): FFT lengths of
+        // 2^9 .. 2^(maxlog+1) points (a real kind's FFT is half its length)
+        const bool cplx = r.kind == K_DFT || r.kind == K_INVDFT;
         bool mid = false;
-        for (int k = 0; k < r.d; ++k)
-            if (r.n[k] >= 1024 && r.n[k] <= (2 << maxlog)) mid = true;
-        if (mid && r.prec == PREC_F32 && (r.kind == K_DFT || r.kind == K_INVDFT))
-            knobs.push_back({&Tune::e32, {0, 1}});
+        for (int k = 0; k < r.d; ++k) {
+            const int fl = (cplx || (r.kind <= K_INVREALDFT && k < r.d - 1)) ? r.n[k] : r.n[k] / 2;
+            if (fl >= 512 && fl <= (2 << maxlog)) mid = true;
+        }
+        if (mid && r.prec == PREC_F32) knobs.push_back({&Tune::e32, {0, 1}});
     }
     knobs.push_back({&Tune::tpw, {256, 128, 512}});
     if (short_row) knobs.push_back({&Tune::wave, {1, 0}});

@@ -42,10 +42,10 @@ static void reg_tiles(KTable& t) {
 #define FNAME CAT(register_f32_k, INST_KIND)
 #endif
 
-// radix-32 tiles for the FP32 complex kinds, 2^10 .. 2^IMAX
+// radix-32 tiles (FP32), FFT lengths 2^9 .. 2^IMAX
 template <int LOGL>
 static void reg_tiles32(KTable& t) {
-#if !INST_F64 && (INST_KIND == 0 || INST_KIND == 1)
+#if !INST_F64
     if constexpr (LOGL <= IMAX) {
         t.tile32[INST_KIND][LOGL] = reinterpret_cast<const void*>(&tile_kernel<IT, LOGL, 32, INST_KIND>);
         reg_tiles32<LOGL + 1>(t);
@@ -57,7 +57,7 @@ static void reg_tiles32(KTable& t) {
 
 void FNAME(KTable& t) {
     reg_tiles<0>(t);
-    reg_tiles32<10>(t);
+    reg_tiles32<9>(t);
     t.maxlog = IMAX;
     t.pre[INST_KIND] = reinterpret_cast<const void*>(&pre_kernel<IT, INST_KIND>);
     t.post[INST_KIND] = reinterpret_cast<const void*>(&post_kernel<IT, INST_KIND>);

@@ -11,7 +11,7 @@ struct KTable {
     const void* tile[K_COUNT][kMaxLog + 1];  // tile_kernel<T, logL, E, kind>
     int tile_E[kMaxLog + 1];                  // points per thread per logL
     // radix-32 variants (32 points per thread, 512-lane work-groups at most):
-    // FP32 complex kinds, logL >= 10 only; null elsewhere
+    // FP32, logL >= 9 only; null elsewhere
     const void* tile32[K_COUNT][kMaxLog + 1];
     int maxlog;                               // largest logL instantiated
     const void* pre[K_COUNT];

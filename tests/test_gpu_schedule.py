@@ -77,6 +77,9 @@ def _cases(prec):
     entries += [("dft", (8, 2048))] * 4 + [("invdft", (1024, 8))] * 4
     entries += [("dft", (fused_n,))] * 260 + [("invdft", (split_n,))] * 40
     entries += [("dct2", (128, 128))] * 8 + [("dst4", (64, 32))] * 8
+    # real kinds whose half-length FFTs reach the radix-32 tiles (rows and columns)
+    entries += [("dct2", (4, 1024)), ("dst3", (1024, 2)), ("dct4", (2, 2048)), ("realdft", (2, 1024)),
+                ("dst2", (1024,)), ("dct3", (2048,)), ("dst4", (1024, 4))] * 2
     entries += [("dft", (split_n // 2,))] * 6 + [("invdft", (split_n // 4,))] * 6
     entries += [("realdft", (1 << 17,))] * 3 + [("realdft", (2 * split_n,))] * 2
     xs = [make_input(k, s, prec, int(rng.integers(1 << 30))) for k, s in entries]
