@@ -7,14 +7,19 @@
 // leading axes (minfft.c:582-597), its inverse the other way round
 // (minfft.c:662-687).  Each pass is one launch of the fused tile kernel when
 // the axis length fits one work-group (2^14 FP32 / 2^13 FP64 points on a
-// 160 KiB-LDS CU), otherwise a two-step (N = N1*N2) decomposition through a
-// scratch buffer.
+// 160 KiB-LDS CU); at and just past that limit a radix-2-split single pass
+// (split2.hpp) may take over; longer axes use a two-step (N = N1*N2)
+// decomposition through a scratch buffer, or for batches of long 1D entries
+// the fused two-step kernel (big.hpp).  Launch shapes come from the schedule
+// in force (tune(): defaults, MINFFT_TUNE, or the autotuner's choice for the
+// plan / batch group).
 #include "engine.hpp"
 
 #include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <string>
 
@@ -206,12 +211,19 @@ TuneScope::~TuneScope() { t_tune = prev; }
 
 // -------------------------------------------------------------- launching
 static std::mutex g_attr_mu;
+static std::map<std::pair<int, const void*>, size_t> g_attr_lds;  // (device, kernel) -> LDS allowed
 
 static int launch_fn(const void* fn, unsigned grid, unsigned block, void** args, size_t lds,
                      hipStream_t st) {
     if (lds > 65536) {
+        // dynamic LDS beyond 64 KiB must be allowed per kernel (once per device)
         std::lock_guard<std::mutex> lk(g_attr_mu);
-        (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        size_t& have = g_attr_lds[{device().id, fn}];
+        if (have < lds) {
+            HIPCHK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),
+                   "hipFuncSetAttribute");
+            have = lds;
+        }
     }
     HIPCHK(hipLaunchKernel(fn, dim3(grid), dim3(block), args, lds, st), "kernel launch");
     return 0;
