@@ -7,8 +7,10 @@
 // Execution: entries with identical (kind, dims) form a group; each group is
 // executed as the same per-axis passes as a single transform, with one extra
 // batch level whose base addresses come from device tables of per-entry
-// offsets, so a group of any size costs one launch per axis (two for an axis
-// longer than one tile).  Each group's launch schedule is picked when the
+// offsets, so a group of any size costs one launch per axis (up to three
+// for an axis longer than one tile: the two steps of the N = N1*N2
+// decomposition plus a real kind's pre or post pass, unless the fused or the
+// radix-2-split kernel covers it in one).  Each group's launch schedule is picked when the
 // plan is made by timing candidates on the device (autotune.cpp).  All
 // groups' launches are captured once into a HIP graph per (in, out, stream)
 // and replayed.

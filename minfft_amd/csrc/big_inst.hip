@@ -1,6 +1,8 @@
-// big_inst.hip -- instances of the fused two-step kernel (big.hpp) and their
-// registry: FP32 with 512-thread work-groups, FP64 with 256 (two work-groups
-// of ~70 KiB LDS per CU where the registers allow).
+// big_inst.hip -- instances of the kernels outside the per-kind tile table,
+// and their registry (bigtable.hpp): the fused two-step kernel (big.hpp; FP32
+// with 512-thread work-groups, FP64 with 256: two work-groups of ~70 KiB LDS
+// per CU where the registers allow), the radix-2-split single-pass kernel
+// (split2.hpp) and the split-FP16 matrix-core 512-point tile (mfma.hpp).
 #include "big.hpp"
 #include "mfma.hpp"
 #include "split2.hpp"
