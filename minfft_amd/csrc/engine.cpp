@@ -325,10 +325,17 @@ static int launch_tile(Prec prec, int kind, int logL, const void* tw, const void
         A.out_mode = out_row ? IO_STAGE_ROW : (out_cf ? IO_STAGE_CF : IO_STAGE_ROW);
         A.cfast = (A.in_mode == IO_STAGE_CF && cf_core) ? 1 : 0;
     }
+    if (split16) {
+        // the MFMA tile keeps a row-mapped core (two transforms per
+        // wavefront): direct I/O for contiguous rows, column I/O staged
+        A.cfast = 0;
+        A.in_mode = in_row ? IO_DIRECT : (in_cf ? IO_STAGE_CF : IO_STAGE_ROW);
+        A.out_mode = out_row ? IO_DIRECT : (out_cf ? IO_STAGE_CF : IO_STAGE_ROW);
+    }
     const bool wants_cf = A.cfast || A.in_mode == IO_STAGE_CF || A.out_mode == IO_STAGE_CF;
     const size_t cs = prec == PREC_F64 ? 16 : 8;
     const int sc = slot_len(L);
-    const int maxT = (prec == PREC_F64 || e32) ? 512 : 1024;  // tile_threads<T, E>() of tile.hpp
+    const int maxT = (prec == PREC_F64 || e32 || split16) ? 512 : 1024;  // the kernels' launch bounds
     int C;
     if (wants_cf) {
         // several columns per tile so that each row segment of the strided
@@ -344,7 +351,6 @@ static int launch_tile(Prec prec, int kind, int logL, const void* tw, const void
     // work-groups (wider row segments are worth less than idle CUs), down to
     // one wavefront per work-group
     while (C > 1 && (C / 2) * TPT >= 64 && (ntr + C - 1) / C < 2ull * (unsigned long long)D.ncu) C >>= 1;
-    if (split16) C = 16;  // the MFMA tile is laid out for 16 transforms, 512 threads
     size_t budget = (size_t)D.lds_max;
     while (C > 1 && (size_t)C * sc * cs > budget) C >>= 1;
     if ((size_t)C * sc * cs > budget) {
@@ -353,6 +359,7 @@ static int launch_tile(Prec prec, int kind, int logL, const void* tw, const void
     }
     // a power of two (the kernels split the lane index with shifts)
     while (C & (C - 1)) C &= C - 1;
+    if (split16 && C < 2) C = 2;  // MFMA operands span a full wavefront: two transforms
     A.C = C;
     A.lgC = ilog2i(C);
     A.sc = sc;
