@@ -271,7 +271,7 @@ static int launch_tile(Prec prec, int kind, int logL, const void* tw, const void
     }
     split16 = split16 && prec == PREC_F32 && logL == 9 && (kind == K_DFT || kind == K_INVDFT) && fs_logN == 0;
     const bool e32 = !split16 && tune().e32 && kt.tile32[kind][logL];
-    const void* fn = split16 ? bigtable().mfma512[kind == K_INVDFT] : (e32 ? kt.tile32[kind][logL] : kt.tile[kind][logL]);
+    const void* fn = e32 ? kt.tile32[kind][logL] : kt.tile[kind][logL];
     const int E = split16 ? 16 : (e32 ? 32 : kt.tile_E[logL]);
     const int L = 1 << logL;
     const int TPT = L / E;
@@ -325,9 +325,18 @@ static int launch_tile(Prec prec, int kind, int logL, const void* tw, const void
         A.out_mode = out_row ? IO_STAGE_ROW : (out_cf ? IO_STAGE_CF : IO_STAGE_ROW);
         A.cfast = (A.in_mode == IO_STAGE_CF && cf_core) ? 1 : 0;
     }
-    if (split16) {
-        // the MFMA tile keeps a row-mapped core (two transforms per
-        // wavefront): direct I/O for contiguous rows, column I/O staged
+    // split-FP16 MFMA tiles: the column variant when both sides are columns
+    // of adjacent transforms (direct I/O, 16 transforms per work-group);
+    // otherwise the row variant (two transforms per wavefront, direct I/O
+    // for contiguous rows, other I/O staged)
+    const bool s16cf = split16 && in_cf && out_cf;
+    if (s16cf) {
+        fn = bigtable().mfma512cf[kind == K_INVDFT];
+        A.cfast = 1;
+        A.in_mode = IO_DIRECT;
+        A.out_mode = IO_DIRECT;
+    } else if (split16) {
+        fn = bigtable().mfma512[kind == K_INVDFT];
         A.cfast = 0;
         A.in_mode = in_row ? IO_DIRECT : (in_cf ? IO_STAGE_CF : IO_STAGE_ROW);
         A.out_mode = out_row ? IO_DIRECT : (out_cf ? IO_STAGE_CF : IO_STAGE_ROW);
@@ -360,6 +369,7 @@ static int launch_tile(Prec prec, int kind, int logL, const void* tw, const void
     // a power of two (the kernels split the lane index with shifts)
     while (C & (C - 1)) C &= C - 1;
     if (split16 && C < 2) C = 2;  // MFMA operands span a full wavefront: two transforms
+    if (s16cf) C = 16;            // the column variant's layout: 16 transforms, 512 lanes
     A.C = C;
     A.lgC = ilog2i(C);
     A.sc = sc;

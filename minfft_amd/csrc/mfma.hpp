@@ -241,4 +241,121 @@ __global__ void __launch_bounds__(512) mfma512_kernel(TileArgs A) {
     stage_out<float, INV ? K_INVDFT : K_DFT, L, TPT>(A, dst, sslot, stl, tw);
 }
 
+// Column variant for strided transforms whose neighbours are adjacent in
+// memory (item stride e, batch stride 1 on both sides): 16 transforms per
+// work-group (512 lanes), and the MFMA columns are the 16 transforms rather
+// than 16 butterflies.  Group (butterfly j, transforms c < 16): lane (c, g)
+// reads x_c[j + 32r], r = 4g + q -- for fixed (g, q) the 16 lanes read 16
+// adjacent points, 128 contiguous bytes -- and wavefront w takes the
+// butterflies j = w + 8i, i < 4.  The scale is per column (transform c,
+// butterfly j): its 16 inputs sit in lanes c + 16g.  Passes 2 and 3 run
+// with the c-fast core mapping (lane t: transform t % 16, thread t / 16),
+// whose owned positions give coalesced direct stores.
+template <bool INV>
+__global__ void __launch_bounds__(512) mfma512_cf_kernel(TileArgs A) {
+    using C2 = float2;
+    constexpr int L = 512, E = 16, TPT = L / E, CT = 16;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    C2* lds = reinterpret_cast<C2*>(smem_raw);
+    const C2* __restrict__ tw = reinterpret_cast<const C2*>(A.tw);
+
+    const int t = threadIdx.x;
+    const unsigned tile = tile_id(A);
+    const int c = t & (CT - 1), tl = t >> 4;  // c-fast core: transform c, thread tl
+    const int g = (t >> 4) & 3, w = t >> 6;     // MFMA operand row block, wavefront
+    const TransformRef me = locate(A, tile * (unsigned)CT + (unsigned)c);
+    C2* slot = lds + (long long)c * A.sc;
+
+    // DFT-16 matrix operands A[k][r] = W_16^{kr}, k = c, r = 4g + q
+    C2 wk[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) wk[q] = twl<float>(tw, 16u + (unsigned)((c * (4 * g + q)) & 15));
+
+    // inputs x_c[j + 32(4g + q)] of the wavefront's butterflies j = w + 8i
+    C2 x[4][4];
+    if (me.valid) {
+        // element j + 32(4g + q) = (w + 128g) + 8i + 32q: one base, immediate steps
+        const long long es = A.in_e;
+        const C2* __restrict__ p0 = reinterpret_cast<const C2*>(A.in) + me.ib + (long long)(w + 128 * g) * es;
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) x[i][q] = p0[(long long)(8 * i + 32 * q) * es];
+    } else {
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) x[i][q] = make_float2(0.f, 0.f);
+    }
+
+    half4_t Ch, Cl, Sh, Sl, NSh, NSl;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        _Float16 h, lo;
+        split_h(wk[q].x, h, lo);  // cos
+        Ch[q] = h;
+        Cl[q] = lo;
+        split_h(-wk[q].y, h, lo);  // sin
+        Sh[q] = h;
+        Sl[q] = lo;
+        NSh[q] = -h;
+        NSl[q] = -lo;
+    }
+    const half4_t Pr_h = INV ? NSh : Sh, Pr_l = INV ? NSl : Sl;  // multiplies xi into Xr
+    const half4_t Pi_h = INV ? Sh : NSh, Pi_l = INV ? Sl : NSl;  // multiplies xr into Xi
+
+    // ---- pass 1 on the matrix cores: output k = 4g + q of butterfly j of
+    // transform c goes to position 16j + k of slot c ----
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int j = w + 8 * i;
+        // column scale: largest |component| of the column -> [2^13, 2^14)
+        float m = 0.f;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) m = fmaxf(m, fmaxf(fabsf(x[i][q].x), fabsf(x[i][q].y)));
+        m = fmaxf(m, __shfl_xor(m, 16));
+        m = fmaxf(m, __shfl_xor(m, 32));
+        int ex = 0;
+        (void)frexpf(m, &ex);
+        const int sh = (m > 0.f) ? 14 - ex : 0;
+        half4_t xrh, xrl, xih, xil;
+        split4(ldexpf(x[i][0].x, sh), ldexpf(x[i][1].x, sh), ldexpf(x[i][2].x, sh), ldexpf(x[i][3].x, sh), xrh, xrl);
+        split4(ldexpf(x[i][0].y, sh), ldexpf(x[i][1].y, sh), ldexpf(x[i][2].y, sh), ldexpf(x[i][3].y, sh), xih, xil);
+        float4_t Xr = {0.f, 0.f, 0.f, 0.f}, Xi = {0.f, 0.f, 0.f, 0.f};
+        Xr = __builtin_amdgcn_mfma_f32_16x16x16f16(Cl, xrh, Xr, 0, 0, 0);
+        Xr = __builtin_amdgcn_mfma_f32_16x16x16f16(Ch, xrl, Xr, 0, 0, 0);
+        Xr = __builtin_amdgcn_mfma_f32_16x16x16f16(Pr_l, xih, Xr, 0, 0, 0);
+        Xr = __builtin_amdgcn_mfma_f32_16x16x16f16(Pr_h, xil, Xr, 0, 0, 0);
+        Xr = __builtin_amdgcn_mfma_f32_16x16x16f16(Ch, xrh, Xr, 0, 0, 0);
+        Xr = __builtin_amdgcn_mfma_f32_16x16x16f16(Pr_h, xih, Xr, 0, 0, 0);
+        Xi = __builtin_amdgcn_mfma_f32_16x16x16f16(Cl, xih, Xi, 0, 0, 0);
+        Xi = __builtin_amdgcn_mfma_f32_16x16x16f16(Ch, xil, Xi, 0, 0, 0);
+        Xi = __builtin_amdgcn_mfma_f32_16x16x16f16(Pi_l, xrh, Xi, 0, 0, 0);
+        Xi = __builtin_amdgcn_mfma_f32_16x16x16f16(Pi_h, xrl, Xi, 0, 0, 0);
+        Xi = __builtin_amdgcn_mfma_f32_16x16x16f16(Ch, xih, Xi, 0, 0, 0);
+        Xi = __builtin_amdgcn_mfma_f32_16x16x16f16(Pi_h, xrh, Xi, 0, 0, 0);
+        // position 16j + 4g + q: padded 17j + 4g + q
+        C2* d = slot + (17 * j + 4 * g);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) d[q] = make_float2(ldexpf(Xr[q], -sh), ldexpf(Xi[q], -sh));
+    }
+    __syncthreads();
+
+    // ---- passes 2 and 3 on the vector ALUs, c-fast core ----
+    C2 v[E];
+    Core<float, 9, E, INV>::read_owned(v, slot, tl);
+    __syncthreads();
+    stockham_pass<float, L, E, 16, 16, INV, false>(v, slot, tl, tw);
+    __syncthreads();
+    Core<float, 9, E, INV>::read_owned(v, slot, tl);
+    stockham_pass<float, L, E, 2, 256, INV, true>(v, slot, tl, tw);
+
+    // ---- output: neighbouring lanes store neighbouring transforms ----
+    if (me.valid) {
+        const Walk<C2> wo(reinterpret_cast<C2*>(A.out) + me.ob, 0, 1, tl, TPT, A.out_e);
+#pragma unroll
+        for (int mm = 0; mm < E; ++mm) wo[mm] = v[mm];
+    }
+}
+
 }  // namespace mfx
