@@ -32,7 +32,7 @@ COMMON = [
 ]
 
 HEADERS = ["fft_core.hpp", "kinds.hpp", "tile.hpp", "ktable.hpp", "engine.hpp", "big.hpp", "bigtable.hpp", "mfma.hpp",
-           "autotune.hpp", "split2.hpp"]
+           "autotune.hpp", "split2.hpp", "rsplit.hpp"]
 
 
 def _newer(target, deps):
@@ -51,6 +51,7 @@ def _units():
             u.append((o, "inst.hip", [f"-DINST_F64={f64}", f"-DINST_KIND={kind}"], "engine"))
     u.append((os.path.join(OBJ, "twiddle.o"), "twiddle.hip", [], "engine"))
     u.append((os.path.join(OBJ, "big_inst.o"), "big_inst.hip", [], "engine"))
+    u.append((os.path.join(OBJ, "rsplit_inst.o"), "rsplit_inst.hip", [], "engine"))
     u.append((os.path.join(OBJ, "engine.o"), "engine.cpp", ["-x", "hip"], "engine"))
     u.append((os.path.join(OBJ, "batch.o"), "batch.cpp", ["-x", "hip"], "engine"))
     u.append((os.path.join(OBJ, "autotune.o"), "autotune.cpp", ["-x", "hip"], "engine"))

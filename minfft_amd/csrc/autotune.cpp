@@ -53,7 +53,7 @@ std::string key_of(Prec prec, int kind, int d, const int* n, unsigned count) {
     return b;
 }
 
-// cache file line: key tpw cfb cfcore wave xcd e32 split split2 fused wpc falt us_default us_best
+// cache file line: key tpw cfb cfcore wave xcd e32 split split2 fused wpc falt rsplit us_default us_best
 void load_cache() {
     if (g_loaded) return;
     g_loaded = true;
@@ -68,7 +68,7 @@ void load_cache() {
         r.sched = base_tune();
         Tune& t = r.sched;
         if (ss >> k >> t.tpw >> t.cfb >> t.cfcore >> t.wave >> t.xcd >> t.e32 >> t.split >> t.split2 >> t.fused >>
-            t.fused_wpc >> t.fused_alt >>
+            t.fused_wpc >> t.fused_alt >> t.rsplit >>
             r.us_default >> r.us_best) {
             r.tuned = true;
             g_cache[k] = r;
@@ -85,7 +85,8 @@ void save_entry(const std::string& k, const TuneResult& r) {
         << ' ' << t.split
         << ' ' << t.split2
         << ' ' << t.fused
-        << ' ' << t.fused_wpc << ' ' << t.fused_alt << ' ' << r.us_default << ' ' << r.us_best << '\n';
+        << ' ' << t.fused_wpc << ' ' << t.fused_alt << ' ' << t.rsplit << ' ' << r.us_default << ' ' << r.us_best
+        << '\n';
 }
 
 struct Knob {
@@ -109,7 +110,7 @@ TuneResult autotune_group(const Workspace& ws, const TuneRequest& r, hipStream_t
     TuneResult res;
     res.sched = base_tune();
     if (!autotune_enabled() || r.count == 0) return res;
-    const std::string key = key_of(r.prec, r.kind, r.d, r.n, r.count);
+    const std::string key = key_of(r.prec, r.kind, r.d, r.n, r.count) + (r.solo ? ".solo" : "");
     {
         std::lock_guard<std::mutex> lk(g_mu);
         load_cache();
@@ -141,6 +142,21 @@ TuneResult autotune_group(const Workspace& ws, const TuneRequest& r, hipStream_t
         knobs.push_back({&Tune::fused_wpc, {2, 1, 4}});
     }
     if (near_limit) knobs.push_back({&Tune::split2, {-1, 1, 2, 3, 0}});
+    if (r.solo && (r.kind == K_DFT || r.kind == K_INVDFT)) {
+        // residue-split kernels for the contiguous (last) axis -- only for a
+        // plan's only group: timed alone they can win, but beside other
+        // groups on concurrent branches their L2-heavy reads and partial-line
+        // writes slow the neighbours more than they gain (on the 8192-entry
+        // uneven FP32 batch: 490 us per step without them, 529-540 us with
+        // the 2^14 and 2^15 groups on them)
+        int lg = 0;
+        while ((1 << (lg + 1)) <= r.n[r.d - 1]) ++lg;
+        std::vector<int> codes = rsplit_codes(r.prec, lg);
+        if (!codes.empty()) {
+            codes.insert(codes.begin(), 0);
+            knobs.insert(knobs.begin(), {&Tune::rsplit, codes});
+        }
+    }
     if (long_axis) knobs.push_back({&Tune::split, {0, -1, 1}});
     if (multi || long_axis) {
         knobs.push_back({&Tune::cfcore, {-1, 0, 1}});
@@ -250,9 +266,11 @@ TuneResult autotune_group(const Workspace& ws, const TuneRequest& r, hipStream_t
 }
 
 std::string describe_tune(const Tune& t) {
-    char b[160];
-    std::snprintf(b, sizeof(b), "tpw=%d cfb=%d cfcore=%d wave=%d xcd=%d e32=%d split=%d split2=%d fused=%d wpc=%d falt=%d",
-                  t.tpw, t.cfb, t.cfcore, t.wave, t.xcd, t.e32, t.split, t.split2, t.fused, t.fused_wpc, t.fused_alt);
+    char b[192];
+    std::snprintf(b, sizeof(b),
+                  "tpw=%d cfb=%d cfcore=%d wave=%d xcd=%d e32=%d split=%d split2=%d fused=%d wpc=%d falt=%d rsplit=%d",
+                  t.tpw, t.cfb, t.cfcore, t.wave, t.xcd, t.e32, t.split, t.split2, t.fused, t.fused_wpc, t.fused_alt,
+                  t.rsplit);
     return b;
 }
 

@@ -18,6 +18,9 @@ struct TuneRequest {
     const long long* d_foff = nullptr;
     void* tbuf = nullptr;
     const long long* d_toff = nullptr;
+    // the group is the plan's only one: no other group runs beside it on a
+    // concurrent graph branch
+    bool solo = false;
 };
 
 struct TuneResult {

@@ -232,6 +232,7 @@ minfft_batch_plan* minfft_mkaux_batch(int count, const minfft_batch_entry* e, in
         r.d_foff = g.d_foff;
         r.tbuf = g.tbuf;
         r.d_toff = g.d_toff;
+        r.solo = p->groups.size() == 1;
         g.tune = autotune_group(p->ws, r, p->stream);
     }
     clear_error();
@@ -282,8 +283,11 @@ static int enqueue_batch(minfft_batch_plan* p, const void* in, void* out, hipStr
     std::vector<long long> load(nb, 0);
     for (auto& g : p->groups) {  // groups are sorted largest first
         int b = (int)(std::min_element(load.begin(), load.end()) - load.begin());
-        // the two-step scratch is shared: long groups stay on branch 0
-        if (g.fbytes) b = 0;
+        // the two-step scratch is shared: groups that use it stay on
+        // branch 0 (1D complex groups whose schedule is single-pass do not)
+        const bool single = g.d == 1 && (g.kind == K_DFT || g.kind == K_INVDFT) &&
+                            c2c_single_pass(p->prec, lg2(g.n[0]), g.tune.sched);
+        if (g.fbytes && !single) b = 0;
         load[b] += group_bytes(g, p->prec);
         TuneScope scope(&g.tune.sched);
         int rc = run_entries(p->ws, p->prec, g.kind, g.d, g.n, in, out, (unsigned)g.ioff.size(), g.d_ioff,

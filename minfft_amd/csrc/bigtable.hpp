@@ -1,4 +1,5 @@
-// bigtable.hpp -- registry of the fused two-step kernel instances.
+// bigtable.hpp -- registry of the kernels outside the per-kind tile table:
+// fused two-step, radix-2-split, residue-split and split-FP16 matrix-core.
 #pragma once
 #include <cstddef>
 
@@ -22,8 +23,14 @@ struct BigTable {
     const void* split2[2][17][2][2];  // [prec][log2 L][variant][inverse]
     int split2_bt[2][17][2];
     size_t split2_lds[2][17];
+    // residue-split single-pass kernels (rsplit.hpp), by log2 N, log2 P - 1
+    // and residues per work-group - 1
+    const void* rsplit[2][17][4][2][2];  // [prec][log2 N][log2 P - 1][B - 1][inverse]
+    int rsplit_bt[2][17][4][2];
+    size_t rsplit_lds[2][17][4][2];
 };
 
 void register_big(BigTable& t);
+void register_rsplit(BigTable& t);
 
 }  // namespace mfx

@@ -85,6 +85,7 @@ static const BigTable& bigtable() {
     std::call_once(g_big_once, [] {
         std::memset(&g_big, 0, sizeof(g_big));
         register_big(g_big);
+        register_rsplit(g_big);
     });
     return g_big;
 }
@@ -156,7 +157,7 @@ bool is_device_pointer(const void* p) {
 // ------------------------------------------------------------ tile tuning
 // Process-wide schedule: defaults, then MINFFT_TUNE="key=value,..." (keys:
 // tpw, cfb, cfcore, wave, xcd, e32, stage, chunk, split, split2, fused, wpc,
-// falt -- the fields of Tune in engine.hpp).
+// falt, rsplit -- the fields of Tune in engine.hpp).
 static Tune g_tune;
 static std::once_flag g_tune_once;
 static thread_local const Tune* t_tune = nullptr;
@@ -191,6 +192,7 @@ bool parse_tune(const char* spec, Tune& t) {
         else if (k == "fused") t.fused = v;
         else if (k == "wpc" && v > 0) t.fused_wpc = v;
         else if (k == "falt") t.fused_alt = v;
+        else if (k == "rsplit") t.rsplit = v;
         else ok = false;
     }
     return ok;
@@ -444,9 +446,80 @@ static int launch_split2(const Workspace& ws, Prec prec, int kind, int logL, con
                      args, bt.split2_lds[pi][logL], st);
 }
 
+// single-pass residue-split launch (rsplit.hpp) for contiguous transforms of
+// length 2^logL, with the schedule's split (tune().rsplit = 10*log2 P + B),
+// if instantiated and its LDS fits; 1 = not applicable
+static int launch_rsplit(const Workspace& ws, Prec prec, int kind, int logL, const void* in, long long in_e,
+                         void* out, long long out_e, const Levels& lv, hipStream_t st) {
+    const int code = tune().rsplit;
+    const int lp = code / 10, b = code % 10;
+    if (code <= 0 || lp < 1 || lp > 4 || b < 1 || b > 2 || in_e != 1 || out_e != 1 || logL > 16) return 1;
+    const BigTable& bt = bigtable();
+    const int pi = prec == PREC_F64 ? 0 : 1;
+    const void* fn = bt.rsplit[pi][logL][lp - 1][b - 1][kind == K_INVDFT];
+    const size_t lds = bt.rsplit_lds[pi][logL][lp - 1][b - 1];
+    if (!fn || lds > (size_t)device().lds_max || ws.tw_log < logL) return 1;
+    TileArgs A;
+    std::memset(&A, 0, sizeof(A));
+    A.in = in;
+    A.out = out;
+    A.tw = ws.tw;
+    A.tw64 = (const double2*)ws.tw64;
+    A.in_e = 1;
+    A.out_e = 1;
+    fill_levels(A, lv);
+    const unsigned long long ntr = lv.count();
+    if (ntr == 0) return 0;
+    // P/B work-groups per transform, transforms in groups of 8 (one per XCD)
+    const unsigned long long G = (1ull << lp) / (unsigned)b;
+    const unsigned long long grid = ((ntr + 7) / 8) * 8 * G;
+    if (grid > 0x7fffffffull) {
+        set_error("batch too large");
+        return -1;
+    }
+    A.ntrans = (unsigned)ntr;
+    A.ntiles = (unsigned)grid;
+    A.C = 1;
+    A.xcd = 1;
+    void* args[] = {&A};
+    return launch_fn(fn, (unsigned)grid, (unsigned)bt.rsplit_bt[pi][logL][lp - 1][b - 1], args, lds, st);
+}
+
+std::vector<int> rsplit_codes(Prec prec, int logN) {
+    std::vector<int> v;
+    if (logN < 1 || logN > 16 || !ensure_device()) return v;
+    const BigTable& bt = bigtable();
+    const int pi = prec == PREC_F64 ? 0 : 1;
+    for (int lp = 1; lp <= 4; ++lp)
+        for (int b = 1; b <= 2; ++b)
+            if (bt.rsplit[pi][logN][lp - 1][b - 1][0] && bt.rsplit_lds[pi][logN][lp - 1][b - 1] <= (size_t)device().lds_max)
+                v.push_back(10 * lp + b);
+    return v;
+}
+
+bool c2c_single_pass(Prec prec, int logL, const Tune& t) {
+    if (!ensure_device()) return false;
+    const int maxlog = device().maxlog[prec];
+    if (logL <= maxlog) return true;
+    if (logL > 16) return false;
+    const BigTable& bt = bigtable();
+    const int pi = prec == PREC_F64 ? 0 : 1;
+    const size_t lmax = (size_t)device().lds_max;
+    const int lp = t.rsplit / 10, b = t.rsplit % 10;
+    if (t.rsplit > 0 && lp >= 1 && lp <= 4 && b >= 1 && b <= 2 && bt.rsplit[pi][logL][lp - 1][b - 1][0] &&
+        bt.rsplit_lds[pi][logL][lp - 1][b - 1] <= lmax)
+        return true;
+    const int sp2 = t.split2 >= 0 ? t.split2 : (prec == PREC_F64 ? 2 : 3);
+    return sp2 >= 1 && logL == maxlog + 1 && bt.split2[pi][logL][0][0] && bt.split2_lds[pi][logL] <= lmax;
+}
+
 static int c2c_axis(const Workspace& ws, Prec prec, int kind, int logL, const void* in, long long in_e,
                     void* out, long long out_e, const Levels& lv, hipStream_t st) {
     const int maxlog = device().maxlog[prec];
+    if (tune().rsplit > 0) {
+        const int rc = launch_rsplit(ws, prec, kind, logL, in, in_e, out, out_e, lv, st);
+        if (rc <= 0) return rc;
+    }
     // radix-2-split kernel: always one past the tile limit (split2 >= 1),
     // at the limit with split2 >= 2, one below it with split2 >= 3
     const int sp2 = tune().split2 >= 0 ? tune().split2 : (prec == PREC_F64 ? 2 : 3);

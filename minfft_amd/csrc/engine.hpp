@@ -80,6 +80,9 @@ struct Tune {
     int fused = 1;      // fused two-step kernel for batches of long 1D entries
     int fused_wpc = 2;  // work-groups per CU of the fused kernel
     int fused_alt = 0;  // use the alternative (smaller) fused work-groups
+    int rsplit = 0;     // residue-split single-pass kernel for contiguous axes:
+                        // 10*log2 P + residues per work-group (e.g. 31 = 8 blocks,
+                        // one residue), where instantiated; 0 = off
 };
 struct Plan {
     unsigned magic = 0x6d667870u;
@@ -103,7 +106,7 @@ struct Plan {
 // the schedule in force on this thread
 const Tune& tune();
 // applies "key=value,..." (the Tune fields: tpw, cfb, wave, xcd, cfcore,
-// e32, stage, chunk, split, split2, fused, wpc, falt) to t; false on an
+// e32, stage, chunk, split, split2, fused, wpc, falt, rsplit) to t; false on an
 // unknown key
 bool parse_tune(const char* spec, Tune& t);
 // the process-wide schedule (defaults + MINFFT_TUNE)
@@ -146,6 +149,13 @@ int needed_tw_log(int kind, int logN);
 int axis_pass(const Workspace& ws, Prec prec, int kind, int logN,
               const void* in, long long in_e, void* out, long long out_e,
               const Levels& lv, hipStream_t st);
+// schedule codes (Tune::rsplit) of the residue-split kernels instantiated
+// for complex transforms of length 2^logN whose LDS fits this device
+std::vector<int> rsplit_codes(Prec prec, int logN);
+// true when a contiguous complex pass of length 2^logL runs as one
+// single-pass launch under schedule t (tile, residue-split or radix-2-split
+// kernel), i.e. touches no two-step scratch
+bool c2c_single_pass(Prec prec, int logL, const Tune& t);
 // scratch (bytes) an axis pass of this shape needs in ws.fbuf / ws.gbuf;
 // in_extent = number of input items addressed (for the mirrored layout)
 size_t axis_scratch_bytes(Prec prec, int kind, int logN, unsigned long long ntrans,

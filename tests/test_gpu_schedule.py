@@ -17,7 +17,8 @@ SPECS = ["", "tpw=128", "tpw=512", "cfb=64", "cfb=256", "xcd=0", "wave=0",
          "e32=1,split2=0,fused=0", "e32=1,cfcore=0",
          "fused=0", "falt=1", "wpc=1", "wpc=4", "split=-1", "split=1",
          "split2=0", "split2=0,split=-1", "split2=0,split=1", "split2=1", "split2=2", "split2=3",
-         "fused=0,split=-1", "fused=0,split=1"]
+         "fused=0,split=-1", "fused=0,split=1",
+         "rsplit=11", "rsplit=12", "rsplit=21", "rsplit=22", "rsplit=31", "rsplit=32", "rsplit=41"]
 
 
 class Packed:
@@ -139,3 +140,44 @@ def test_autotuner_tunes_large_groups(oracle64):
     with pytest.raises(mf.MinfftError):
         pk.batch.schedule(99, "")
     pk.batch.close()
+
+
+@pytest.mark.parametrize("prec", [32, 64])
+def test_residue_split_every_code(oracle64, prec):
+    """Every residue-split kernel (rsplit.hpp) on forward and inverse entries of
+    each length it serves, with the entries one complex item off the 16-byte
+    grid (so the FP32 paired store takes its unaligned branch on odd entries)
+    and counts that are not multiples of the 8-transform grid groups."""
+    import torch
+    rs = 8 if prec == 64 else 4
+    cs = 2 * rs
+    cdt = np.complex128 if prec == 64 else np.complex64
+    rng = np.random.default_rng(11 + prec)
+    for logn in (14, 15, 16):
+        n = 1 << logn
+        kinds = ["dft"] * 11 + ["invdft"] * 5
+        xs = [(rng.uniform(-1, 1, n) + 1j * rng.uniform(-1, 1, n)).astype(cdt) for _ in kinds]
+        offs = [(i * (n + 1) + 1) * cs for i in range(len(kinds))]
+        total = offs[-1] + n * cs + cs
+        hin = np.zeros(total, dtype=np.uint8)
+        for x, o in zip(xs, offs):
+            hin[o:o + n * cs] = np.frombuffer(x.tobytes(), dtype=np.uint8)
+        din = torch.from_numpy(hin).cuda()
+        dout = torch.zeros(total, dtype=torch.uint8, device="cuda")
+        batch = mf.Batch([(k, (n,), o, o) for k, o in zip(kinds, offs)],
+                         mf.PREC_FP64 if prec == 64 else mf.PREC_FP32)
+        refs = [oracle64.run(k, x.astype(np.complex128), (n,)) for k, x in zip(kinds, xs)]
+        for code in (11, 12, 21, 22, 31, 32, 41, 42):
+            for g in range(2):
+                batch.schedule(g, f"rsplit={code}")
+            print(f"[case] rsplit={code} n=2^{logn} prec={prec}", flush=True)
+            dout.zero_()
+            batch.run(din.data_ptr(), dout.data_ptr(), None)
+            torch.cuda.synchronize()
+            hout = dout.cpu().numpy()
+            for i, (k, o) in enumerate(zip(kinds, offs)):
+                y = hout[o:o + n * cs].view(cdt)
+                bar = tol64((n,)) if prec == 64 else TOL32
+                e = rel_l2(y, refs[i])
+                assert e <= bar, (code, logn, i, k, e)
+        batch.close()
