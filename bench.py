@@ -371,7 +371,7 @@ def config_c4(a, torch, mf):
     ms32 = _time_enqueue(torch, lambda: a32.enqueue("dft", x32, y32, sh), a.steps, a.warmup, st)
     x64 = x32.to(torch.complex128)
     y64 = torch.empty_like(x64)
-    ms64 = _time_enqueue(torch, lambda: a64.enqueue("dft", x64, y64, sh), max(3, a.steps // 2), 1, st)
+    ms64 = _time_enqueue(torch, lambda: a64.enqueue("dft", x64, y64, sh), a.steps, a.warmup, st)
     nrm = torch.linalg.vector_norm(y64)
     d = (ys.to(torch.complex128) - y64)
     rel = float(torch.linalg.vector_norm(d) / nrm)
@@ -430,7 +430,7 @@ def config_c5(a, torch, mf):
     x64 = x.to(torch.float64)
     y64 = torch.empty_like(x64)
     p64 = mf.Batch([(kinds[i % 3], (n, n), i * img * 2, i * img * 2) for i in range(count)], mf.PREC_FP64)
-    ms64 = _time_enqueue(torch, lambda: p64.run(x64.data_ptr(), y64.data_ptr(), sh), max(3, a.steps // 2), 1, st)
+    ms64 = _time_enqueue(torch, lambda: p64.run(x64.data_ptr(), y64.data_ptr(), sh), a.steps, a.warmup, st)
     p64.close()
     N = n * n * count
     alg = 2 * N * 4
@@ -449,7 +449,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison leg")
     ap.add_argument("--copy-probe", action="store_true",
@@ -490,7 +490,7 @@ def main():
 
     fp64 = None
     if not a.no_fp64:
-        ms64, xi64, xo64, _, p64 = gpu_batch(sizes, mf.PREC_FP64, max(a.steps // 2, 3), 2, torch, mf)
+        ms64, xi64, xo64, _, p64 = gpu_batch(sizes, mf.PREC_FP64, a.steps, a.warmup, torch, mf)
         fp64 = {"ms_per_step": ms64, "gflops": flops / (ms64 * 1e-3) / 1e9,
                 "hbm_gbs_algorithmic": 2 * alg_bytes / (ms64 * 1e-3) / 1e9}
         log(f"[bench] fp64 batch: {ms64:.4f} ms/step")
