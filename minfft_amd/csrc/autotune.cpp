@@ -158,7 +158,7 @@ TuneResult autotune_group(const Workspace& ws, const TuneRequest& r, hipStream_t
         }
     }
     if (long_axis) knobs.push_back({&Tune::split, {0, -1, 1}});
-    if (multi) knobs.push_back({&Tune::pipe, {0, 1}});
+    knobs.push_back({&Tune::pipe, {0, 1}});
     if (multi || long_axis) {
         knobs.push_back({&Tune::cfcore, {-1, 0, 1}});
         knobs.push_back({&Tune::cfb, {128, 64, 256, 32}});

@@ -404,11 +404,12 @@ static int launch_tile(Prec prec, int kind, int logL, const void* tw, const void
     void* args[] = {&A};
     const unsigned block = (unsigned)(C * TPT);
     const size_t lds = (size_t)C * sc * cs;
-    // persistent pipelined tiles for staged column input: as many
-    // work-groups as stay resident, a multiple of the XCD count (tile.hpp:
-    // tile_kernel_p)
+    // persistent pipelined tiles for staged column input or direct complex
+    // input: as many work-groups as stay resident, a multiple of the XCD
+    // count (tile.hpp: tile_kernel_p)
     const void* fnp = kt.tilep[kind][logL];
-    if (tn.pipe && fnp && !e32 && !split16 && A.in_mode == IO_STAGE_CF && tiles > (unsigned long long)D.ncu) {
+    const bool pipe_in = A.in_mode == IO_STAGE_CF || (cplx && A.in_mode == IO_DIRECT);
+    if (tn.pipe && fnp && !e32 && !split16 && !s16cf && pipe_in && tiles > (unsigned long long)D.ncu) {
         unsigned long long g = (unsigned long long)occupancy(fnp, block, lds) * (unsigned long long)D.ncu;
         if (g >= (unsigned long long)D.nxcd) g -= g % (unsigned long long)D.nxcd;
         if (g < tiles) return launch_fn(fnp, (unsigned)g, block, args, lds, st);

@@ -610,6 +610,24 @@ template <typename T> constexpr int max_threads() { return sizeof(T) == 8 ? 512 
 // radix-32 tiles (E = 32) hold twice the points per lane: at most 512 lanes
 template <typename T, int E> constexpr int tile_threads() { return E >= 32 ? 512 : max_threads<T>(); }
 
+// Complex input of transform r at the owned positions tl + m*TPT (direct
+// input of the persistent kernel; every position is assigned on both paths)
+template <typename T, int L, int E>
+__device__ __forceinline__ void direct_load(const TileArgs& A, const TransformRef& r, int tl, c2<T>* v) {
+    constexpr int TPT = L / E;
+    const c2<T>* __restrict__ in = reinterpret_cast<const c2<T>*>(A.in) + r.ib;
+    if (A.in_e == 1) {
+#pragma unroll
+        for (int m = 0; m < E; ++m) v[m] = in[tl + m * TPT];
+    } else {
+        const long long es = A.in_e;
+        const c2<T>* __restrict__ b = in + (long long)tl * es;
+        const long long step = es * TPT;
+#pragma unroll
+        for (int m = 0; m < E; ++m) v[m] = b[m * step];
+    }
+}
+
 // One tile of the pass.  PIPE (persistent kernel): the staged input of this
 // tile is already in R; its LDS stores are followed by the loads of tile
 // `next` (~0u: none) into R, which stay in flight through this tile's FFT.
@@ -652,7 +670,14 @@ __device__ __forceinline__ void tile_body(const TileArgs& A, const unsigned tile
     // (direct input only for the complex kinds: the real kinds always stage,
     // and compiling a dead direct path costs them registers)
     constexpr bool CPLX = (KIND == K_DFT || KIND == K_INVDFT);
-    if (!PIPE && CPLX && A.in_mode == IO_DIRECT) {
+    if (PIPE && CPLX && A.in_mode == IO_DIRECT) {
+        // this tile's owned positions were loaded into R; load the next
+        // tile's (unconditionally, as for staged input)
+#pragma unroll
+        for (int m = 0; m < E; ++m) v[m] = R.c[m];
+        const TransformRef nme = locate(A, (next != ~0u ? next : 0u) * (unsigned)C + (unsigned)c);
+        direct_load<T, L, E>(A, nme, tl, R.c);
+    } else if (CPLX && A.in_mode == IO_DIRECT) {
         // complex input read straight into the owned positions
         const C2* __restrict__ in = reinterpret_cast<const C2*>(A.in) + me.ib;
         if (me.valid) {
@@ -767,12 +792,14 @@ __global__ void __launch_bounds__((tile_threads<T, E>())) tile_kernel(TileArgs A
     tile_body<T, LOGL, E, KIND, false>(A, tile_id(A), ~0u, R);
 }
 
-// Persistent form for passes with staged input (column passes): work-group
+// Persistent form for passes with staged (column) input or direct complex
+// input: work-group
 // g runs the tiles of indices g, g + G, g + 2G, ... (G = grid size, a
 // multiple of the XCD count, so that the XCD tile order keeps each
 // work-group inside its XCD's tile range) and loads the next tile's input
 // into registers before transforming the current one, so that its memory
-// latency hides behind the FFT even at one work-group per CU.
+// latency hides behind the FFT even at one work-group per CU.  The launch
+// keeps the grid below the tile count.
 template <typename T, int LOGL, int E, int KIND>
 __global__ void __launch_bounds__((tile_threads<T, E>())) tile_kernel_p(TileArgs A) {
     constexpr int L = 1 << LOGL, TPT = L / E;
@@ -781,9 +808,15 @@ __global__ void __launch_bounds__((tile_threads<T, E>())) tile_kernel_p(TileArgs
     unsigned b = blockIdx.x;
     if (b >= A.ntiles) return;  // uniform over the work-group
     unsigned tile = tile_of(A, b);
-    {
+    const int t = threadIdx.x;
+    if (kind_complex_in(KIND) && (KIND == K_DFT || KIND == K_INVDFT) && A.in_mode == IO_DIRECT) {
+        // the core's lane mapping of tile_body
+        const int c = A.cfast ? (t & (A.C - 1)) : t / TPT;
+        const int tl = A.cfast ? (t >> A.lgC) : t - (t / TPT) * TPT;
+        const TransformRef me = locate(A, tile * (unsigned)A.C + (unsigned)c);
+        if constexpr (KIND == K_DFT || KIND == K_INVDFT) direct_load<T, L, E>(A, me, tl, R.c);
+    } else {
         // the staging lane mapping of tile_body
-        const int t = threadIdx.x;
         const bool rowm = (A.in_mode == IO_STAGE_ROW);
         const int sc_ = rowm ? t / TPT : (t & (A.C - 1));
         const int stl = rowm ? t - sc_ * TPT : (t >> A.lgC);

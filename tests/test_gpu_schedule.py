@@ -186,9 +186,11 @@ def test_residue_split_every_code(oracle64, prec):
 
 @pytest.mark.parametrize("prec", [32, 64])
 def test_pipelined_column_tiles_every_kind(oracle64, prec):
-    """The persistent pipelined tile kernel (tile_kernel_p) on the column
-    passes of every kind: 2D entries with more column tiles than CUs, so the
-    kernel loops and prefetches; with and without the XCD tile order."""
+    """The persistent pipelined tile kernel (tile_kernel_p): staged column
+    input of every kind and direct complex input (rows, c-fast columns, 1D
+    entries at and below the single-tile limit), with more tiles than CUs so
+    that the kernel loops and prefetches; with and without the XCD tile
+    order."""
     kinds = ["dft", "invdft", "realdft", "invrealdft", "dct2", "dst2", "dct3", "dst3", "dct4", "dst4"]
     shape = (256, 256)
     cnt = 40
@@ -198,15 +200,22 @@ def test_pipelined_column_tiles_every_kind(oracle64, prec):
         for _ in range(cnt):
             entries.append((k, shape))
             xs.append(make_input(k, shape, prec, int(rng.integers(1 << 30)), oracle64))
+    lim = 1 << (14 if prec == 32 else 13)
+    for k, n in (("dft", lim), ("invdft", lim // 2)):
+        for _ in range(300):
+            entries.append((k, (n,)))
+            xs.append(make_input(k, (n,), prec, int(rng.integers(1 << 30))))
     pk = Packed(entries, xs, prec)
     ngroups = pk.batch.describe().count("\n")
     check = [i * cnt + j for i in range(len(kinds)) for j in (0, cnt // 2, cnt - 1)]
+    base = len(kinds) * cnt
+    check += [base, base + 150, base + 299, base + 300, base + 451, base + 599]
     refs = {}
     for i in check:
         kind, shp = entries[i]
         x = xs[i].astype(np.complex128 if is_complex_in(kind) else np.float64)
         refs[i] = oracle64.run(kind, x, shp)
-    for spec in ("pipe=1,cfcore=0", "pipe=1,cfcore=0,xcd=0"):
+    for spec in ("pipe=1", "pipe=1,cfcore=0", "pipe=1,cfcore=0,xcd=0", "pipe=1,split2=0"):
         for g in range(ngroups):
             pk.batch.schedule(g, spec)
         print(f"[case] schedule '{spec}' prec={prec}", flush=True)
