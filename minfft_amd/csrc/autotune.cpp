@@ -53,7 +53,7 @@ std::string key_of(Prec prec, int kind, int d, const int* n, unsigned count) {
     return b;
 }
 
-// cache file line: key tpw cfb cfcore wave xcd e32 split split2 fused wpc falt pipe rsplit us_default us_best
+// cache file line: key tpw cfb cfcore wave xcd e32 split split2 fused wpc falt rsplit us_default us_best
 void load_cache() {
     if (g_loaded) return;
     g_loaded = true;
@@ -68,7 +68,7 @@ void load_cache() {
         r.sched = base_tune();
         Tune& t = r.sched;
         if (ss >> k >> t.tpw >> t.cfb >> t.cfcore >> t.wave >> t.xcd >> t.e32 >> t.split >> t.split2 >> t.fused >>
-            t.fused_wpc >> t.fused_alt >> t.pipe >> t.rsplit >>
+            t.fused_wpc >> t.fused_alt >> t.rsplit >>
             r.us_default >> r.us_best) {
             r.tuned = true;
             g_cache[k] = r;
@@ -85,7 +85,7 @@ void save_entry(const std::string& k, const TuneResult& r) {
         << ' ' << t.split
         << ' ' << t.split2
         << ' ' << t.fused
-        << ' ' << t.fused_wpc << ' ' << t.fused_alt << ' ' << t.pipe << ' ' << t.rsplit << ' ' << r.us_default << ' ' << r.us_best
+        << ' ' << t.fused_wpc << ' ' << t.fused_alt << ' ' << t.rsplit << ' ' << r.us_default << ' ' << r.us_best
         << '\n';
 }
 
@@ -158,7 +158,6 @@ TuneResult autotune_group(const Workspace& ws, const TuneRequest& r, hipStream_t
         }
     }
     if (long_axis) knobs.push_back({&Tune::split, {0, -1, 1}});
-    knobs.push_back({&Tune::pipe, {0, 1}});
     if (multi || long_axis) {
         knobs.push_back({&Tune::cfcore, {-1, 0, 1}});
         knobs.push_back({&Tune::cfb, {128, 64, 256, 32}});
@@ -269,10 +268,9 @@ TuneResult autotune_group(const Workspace& ws, const TuneRequest& r, hipStream_t
 std::string describe_tune(const Tune& t) {
     char b[192];
     std::snprintf(b, sizeof(b),
-                  "tpw=%d cfb=%d cfcore=%d wave=%d xcd=%d e32=%d split=%d split2=%d fused=%d wpc=%d falt=%d pipe=%d "
-                  "rsplit=%d",
+                  "tpw=%d cfb=%d cfcore=%d wave=%d xcd=%d e32=%d split=%d split2=%d fused=%d wpc=%d falt=%d rsplit=%d",
                   t.tpw, t.cfb, t.cfcore, t.wave, t.xcd, t.e32, t.split, t.split2, t.fused, t.fused_wpc, t.fused_alt,
-                  t.pipe, t.rsplit);
+                  t.rsplit);
     return b;
 }
 

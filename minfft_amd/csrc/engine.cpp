@@ -22,7 +22,6 @@
 #include <map>
 #include <mutex>
 #include <string>
-#include <tuple>
 
 #include "bigtable.hpp"
 #include "ktable.hpp"
@@ -158,7 +157,7 @@ bool is_device_pointer(const void* p) {
 // ------------------------------------------------------------ tile tuning
 // Process-wide schedule: defaults, then MINFFT_TUNE="key=value,..." (keys:
 // tpw, cfb, cfcore, wave, xcd, e32, stage, chunk, split, split2, fused, wpc,
-// falt, pipe, rsplit -- the fields of Tune in engine.hpp).
+// falt, rsplit -- the fields of Tune in engine.hpp).
 static Tune g_tune;
 static std::once_flag g_tune_once;
 static thread_local const Tune* t_tune = nullptr;
@@ -194,7 +193,6 @@ bool parse_tune(const char* spec, Tune& t) {
         else if (k == "wpc" && v > 0) t.fused_wpc = v;
         else if (k == "falt") t.fused_alt = v;
         else if (k == "rsplit") t.rsplit = v;
-        else if (k == "pipe") t.pipe = v;
         else ok = false;
     }
     return ok;
@@ -216,24 +214,6 @@ TuneScope::~TuneScope() { t_tune = prev; }
 // -------------------------------------------------------------- launching
 static std::mutex g_attr_mu;
 static std::map<std::pair<int, const void*>, size_t> g_attr_lds;  // (device, kernel) -> LDS allowed
-
-// resident work-groups per CU of a kernel at this launch shape (cached)
-static int occupancy(const void* fn, unsigned block, size_t lds) {
-    static std::mutex mu;
-    static std::map<std::tuple<int, const void*, unsigned, size_t>, int> cache;
-    std::lock_guard<std::mutex> lk(mu);
-    auto key = std::make_tuple(device().id, fn, block, lds);
-    auto it = cache.find(key);
-    if (it != cache.end()) return it->second;
-    if (lds > 65536) (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    int n = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, (int)block, lds) != hipSuccess || n < 1) {
-        (void)hipGetLastError();
-        n = 1;
-    }
-    cache[key] = n;
-    return n;
-}
 
 static int launch_fn(const void* fn, unsigned grid, unsigned block, void** args, size_t lds,
                      hipStream_t st) {
@@ -402,19 +382,7 @@ static int launch_tile(Prec prec, int kind, int logL, const void* tw, const void
     // I/O keeps its work-group barriers (tile.hpp)
     A.wave = (tn.wave && !A.cfast) ? 1 : 0;
     void* args[] = {&A};
-    const unsigned block = (unsigned)(C * TPT);
-    const size_t lds = (size_t)C * sc * cs;
-    // persistent pipelined tiles for staged column input or direct complex
-    // input: as many work-groups as stay resident, a multiple of the XCD
-    // count (tile.hpp: tile_kernel_p)
-    const void* fnp = kt.tilep[kind][logL];
-    const bool pipe_in = A.in_mode == IO_STAGE_CF || (cplx && A.in_mode == IO_DIRECT);
-    if (tn.pipe && fnp && !e32 && !split16 && !s16cf && pipe_in && tiles > (unsigned long long)D.ncu) {
-        unsigned long long g = (unsigned long long)occupancy(fnp, block, lds) * (unsigned long long)D.ncu;
-        if (g >= (unsigned long long)D.nxcd) g -= g % (unsigned long long)D.nxcd;
-        if (g < tiles) return launch_fn(fnp, (unsigned)g, block, args, lds, st);
-    }
-    return launch_fn(fn, (unsigned)tiles, block, args, lds, st);
+    return launch_fn(fn, (unsigned)tiles, (unsigned)(C * TPT), args, (size_t)C * sc * cs, st);
 }
 
 static int launch_elementwise(const void* fn, TileArgs& A, int L, long long total, hipStream_t st) {

@@ -80,9 +80,6 @@ struct Tune {
     int fused = 1;      // fused two-step kernel for batches of long 1D entries
     int fused_wpc = 2;  // work-groups per CU of the fused kernel
     int fused_alt = 0;  // use the alternative (smaller) fused work-groups
-    int pipe = 0;       // persistent tile kernel with the next tile's staged input
-                        // loaded during the current tile's FFT, for passes with
-                        // staged input (column passes)
     int rsplit = 0;     // residue-split single-pass kernel for contiguous axes:
                         // 10*log2 P + residues per work-group (e.g. 31 = 8 blocks,
                         // one residue), where instantiated; 0 = off
@@ -109,7 +106,7 @@ struct Plan {
 // the schedule in force on this thread
 const Tune& tune();
 // applies "key=value,..." (the Tune fields: tpw, cfb, wave, xcd, cfcore,
-// e32, stage, chunk, split, split2, fused, wpc, falt, pipe, rsplit) to t; false on an
+// e32, stage, chunk, split, split2, fused, wpc, falt, rsplit) to t; false on an
 // unknown key
 bool parse_tune(const char* spec, Tune& t);
 // the process-wide schedule (defaults + MINFFT_TUNE)

@@ -55,21 +55,8 @@ static void reg_tiles32(KTable& t) {
 #endif
 }
 
-// persistent pipelined tiles, FFT lengths 2^6 .. 2^14 (complex kinds) or
-// 2^6 .. 2^12 (real kinds: their column passes)
-template <int LOGL>
-static void reg_tilesp(KTable& t) {
-    constexpr bool cplx = INST_KIND == K_DFT || INST_KIND == K_INVDFT;
-    if constexpr (LOGL <= IMAX && (LOGL <= 12 || cplx)) {
-        constexpr int E = default_E(LOGL);
-        t.tilep[INST_KIND][LOGL] = reinterpret_cast<const void*>(&tile_kernel_p<IT, LOGL, E, INST_KIND>);
-        reg_tilesp<LOGL + 1>(t);
-    }
-}
-
 void FNAME(KTable& t) {
     reg_tiles<0>(t);
-    reg_tilesp<6>(t);
     reg_tiles32<9>(t);
     t.maxlog = IMAX;
     t.pre[INST_KIND] = reinterpret_cast<const void*>(&pre_kernel<IT, INST_KIND>);

@@ -13,10 +13,6 @@ struct KTable {
     // radix-32 variants (32 points per thread, 512-lane work-groups at most):
     // FP32, logL >= 9 only; null elsewhere
     const void* tile32[K_COUNT][kMaxLog + 1];
-    // persistent tiles with pipelined staged input (tile_kernel_p), default
-    // E, for the lengths of column passes where one work-group fills a CU's
-    // LDS: null elsewhere
-    const void* tilep[K_COUNT][kMaxLog + 1];
     int maxlog;                               // largest logL instantiated
     const void* pre[K_COUNT];
     const void* post[K_COUNT];

@@ -88,17 +88,17 @@ __device__ __forceinline__ TransformRef locate(const TileArgs& A, unsigned b) {
     return r;
 }
 
-// Remap a work-group index so that consecutive tiles (neighbouring columns
-// in memory) land on the same XCD and share its L2: work-groups are
+// Remap the hardware work-group id so that consecutive tiles (neighbouring
+// columns in memory) land on the same XCD and share its L2: work-groups are
 // dispatched round-robin over the XCDs.
-__device__ __forceinline__ unsigned tile_of(const TileArgs& A, unsigned g) {
+__device__ __forceinline__ unsigned tile_id(const TileArgs& A) {
+    unsigned g = blockIdx.x;
     const unsigned X = (unsigned)A.xcd;
     if (X <= 1) return g;
     const unsigned per = A.ntiles / X;
     if (g >= per * X) return g;  // tail keeps identity
     return (g % X) * per + g / X;
 }
-__device__ __forceinline__ unsigned tile_id(const TileArgs& A) { return tile_of(A, blockIdx.x); }
 
 // Items a + s*(stl + k*TPT) (s = +-1) of a strided row, k = 0, 1, ...: one
 // 64-bit base per family of positions and a uniform step, instead of a
@@ -188,66 +188,6 @@ __device__ __forceinline__ void stage_in(const TileArgs& A, const TransformRef& 
         for (int k = 0; k < K; ++k) {
             if constexpr (TPT % 32 == 0) d0[k * (TPT + TPT / 16)] = r[k];
             else rs[ridx(stl + k * TPT)] = r[k];
-        }
-    }
-}
-
-// One lane's share (items stl, stl + TPT, ...) of a transform's staged
-// input, held in registers between its global loads (stage_load) and its LDS
-// stores (stage_put): the persistent tile kernel issues the loads of its
-// next tile before the FFT of the current one.  Item-by-item loads only (the
-// pipelined kernel serves column passes); stage_in above keeps the fused
-// load-and-store form, which lets the compiler keep only one of its vector
-// and scalar variants live.
-template <typename T, int KIND, int L, int TPT>
-struct StageRegs {
-    static constexpr bool CIN = kind_complex_in(KIND);
-    c2<T> c[CIN ? L / TPT : 1];  // complex items
-    T r[CIN ? 1 : 2 * L / TPT];  // real items
-    c2<T> extra;                 // inverse real DFT: item L (lane stl == 0)
-};
-
-template <typename T, int KIND, int L, int TPT>
-__device__ __forceinline__ void stage_load(const TileArgs& A, const TransformRef& src, int stl,
-                                           StageRegs<T, KIND, L, TPT>& R) {
-    using C2 = c2<T>;
-    if constexpr (kind_complex_in(KIND)) {
-        const C2* __restrict__ in = reinterpret_cast<const C2*>(A.in) + src.ib;
-        const long long es = A.in_e;
-        const Walk<const C2> wi(in, 0, 1, stl, TPT, es);
-#pragma unroll
-        for (int k = 0; k < L / TPT; ++k) R.c[k] = wi[k];
-        if constexpr (kind_nin(KIND, L) > L) {
-            if (stl == 0) R.extra = in[(long long)L * es];
-        }
-    } else {
-        const T* __restrict__ in = reinterpret_cast<const T*>(A.in) + src.ib;
-        const Walk<const T> wi(in, 0, 1, stl, TPT, A.in_e);
-#pragma unroll
-        for (int k = 0; k < 2 * L / TPT; ++k) R.r[k] = wi[k];
-    }
-}
-
-template <typename T, int KIND, int L, int TPT>
-__device__ __forceinline__ void stage_put(const StageRegs<T, KIND, L, TPT>& R, c2<T>* sslot, int stl) {
-    using C2 = c2<T>;
-    if constexpr (kind_complex_in(KIND)) {
-        C2* d0 = sslot + lpad(stl);
-#pragma unroll
-        for (int k = 0; k < L / TPT; ++k) {
-            if constexpr (TPT % 16 == 0) d0[k * (TPT + TPT / 16)] = R.c[k];
-            else sslot[lpad(stl + k * TPT)] = R.c[k];
-        }
-        if constexpr (kind_nin(KIND, L) > L) {
-            if (stl == 0) sslot[lpad(L)] = R.extra;
-        }
-    } else {
-        T* rs = reinterpret_cast<T*>(sslot);
-        T* d0 = rs + ridx(stl);
-#pragma unroll
-        for (int k = 0; k < 2 * L / TPT; ++k) {
-            if constexpr (TPT % 32 == 0) d0[k * (TPT + TPT / 16)] = R.r[k];
-            else rs[ridx(stl + k * TPT)] = R.r[k];
         }
     }
 }
@@ -610,30 +550,8 @@ template <typename T> constexpr int max_threads() { return sizeof(T) == 8 ? 512 
 // radix-32 tiles (E = 32) hold twice the points per lane: at most 512 lanes
 template <typename T, int E> constexpr int tile_threads() { return E >= 32 ? 512 : max_threads<T>(); }
 
-// Complex input of transform r at the owned positions tl + m*TPT (direct
-// input of the persistent kernel; every position is assigned on both paths)
-template <typename T, int L, int E>
-__device__ __forceinline__ void direct_load(const TileArgs& A, const TransformRef& r, int tl, c2<T>* v) {
-    constexpr int TPT = L / E;
-    const c2<T>* __restrict__ in = reinterpret_cast<const c2<T>*>(A.in) + r.ib;
-    if (A.in_e == 1) {
-#pragma unroll
-        for (int m = 0; m < E; ++m) v[m] = in[tl + m * TPT];
-    } else {
-        const long long es = A.in_e;
-        const c2<T>* __restrict__ b = in + (long long)tl * es;
-        const long long step = es * TPT;
-#pragma unroll
-        for (int m = 0; m < E; ++m) v[m] = b[m * step];
-    }
-}
-
-// One tile of the pass.  PIPE (persistent kernel): the staged input of this
-// tile is already in R; its LDS stores are followed by the loads of tile
-// `next` (~0u: none) into R, which stay in flight through this tile's FFT.
-template <typename T, int LOGL, int E, int KIND, bool PIPE>
-__device__ __forceinline__ void tile_body(const TileArgs& A, const unsigned tile, const unsigned next,
-                                          StageRegs<T, KIND, (1 << LOGL), (1 << LOGL) / E>& R) {
+template <typename T, int LOGL, int E, int KIND>
+__global__ void __launch_bounds__((tile_threads<T, E>())) tile_kernel(TileArgs A) {
     using C2 = c2<T>;
     constexpr int L = 1 << LOGL;
     constexpr int TPT = L / E;
@@ -646,11 +564,9 @@ __device__ __forceinline__ void tile_body(const TileArgs& A, const unsigned tile
     C2* lds = reinterpret_cast<C2*>(smem_raw);
     const C2* __restrict__ tw = reinterpret_cast<const C2*>(A.tw);
 
-    // persistent loop: the lane index is re-read each tile, so that the
-    // compiler does not hoist lane-dependent twiddle reads out of the loop
-    // (they would pin dozens of registers across the iterations)
-    const int t = PIPE ? opaque((int)threadIdx.x) : (int)threadIdx.x;
+    const int t = threadIdx.x;
     const int C = A.C;
+    const unsigned tile = tile_id(A);
     // the two lane mappings
     const int c_row = t / TPT, tl_row = t - c_row * TPT;
     const int c_cf = t & (C - 1), tl_cf = t >> A.lgC;
@@ -670,14 +586,7 @@ __device__ __forceinline__ void tile_body(const TileArgs& A, const unsigned tile
     // (direct input only for the complex kinds: the real kinds always stage,
     // and compiling a dead direct path costs them registers)
     constexpr bool CPLX = (KIND == K_DFT || KIND == K_INVDFT);
-    if (PIPE && CPLX && A.in_mode == IO_DIRECT) {
-        // this tile's owned positions were loaded into R; load the next
-        // tile's (unconditionally, as for staged input)
-#pragma unroll
-        for (int m = 0; m < E; ++m) v[m] = R.c[m];
-        const TransformRef nme = locate(A, (next != ~0u ? next : 0u) * (unsigned)C + (unsigned)c);
-        direct_load<T, L, E>(A, nme, tl, R.c);
-    } else if (CPLX && A.in_mode == IO_DIRECT) {
+    if (CPLX && A.in_mode == IO_DIRECT) {
         // complex input read straight into the owned positions
         const C2* __restrict__ in = reinterpret_cast<const C2*>(A.in) + me.ib;
         if (me.valid) {
@@ -701,17 +610,7 @@ __device__ __forceinline__ void tile_body(const TileArgs& A, const unsigned tile
         const int stl = rowm ? tl_row : tl_cf;
         const TransformRef src = (sc_ == c) ? me : locate(A, tile * (unsigned)C + (unsigned)sc_);
         C2* sslot = lds + (long long)sc_ * A.sc;
-        if constexpr (PIPE) {
-            if (src.valid) stage_put<T, KIND, L, TPT>(R, sslot, stl);
-            // unconditional loads (a conditional refill would keep the old
-            // and the new values of R live at once): past the last tile, or
-            // past the last transform, locate() yields transform 0, which is
-            // always readable, and the values are never used
-            const TransformRef nsrc = locate(A, (next != ~0u ? next : 0u) * (unsigned)C + (unsigned)sc_);
-            stage_load<T, KIND, L, TPT>(A, nsrc, stl, R);
-        } else {
-            if (src.valid) stage_in<T, KIND, L, TPT>(A, src, sslot, stl);
-        }
+        if (src.valid) stage_in<T, KIND, L, TPT>(A, src, sslot, stl);
         tile_sync(wv_in);
         const T* rslot = reinterpret_cast<const T*>(slot);
         auto X = [&](int i) -> T { return rslot[ridx(i)]; };
@@ -783,54 +682,6 @@ __device__ __forceinline__ void tile_body(const TileArgs& A, const unsigned tile
             post_real<T, KIND, L, E, TPT>(A, dst, sslot, stl, tw);
         else
             stage_out<T, KIND, L, TPT>(A, dst, sslot, stl, tw);
-    }
-}
-
-template <typename T, int LOGL, int E, int KIND>
-__global__ void __launch_bounds__((tile_threads<T, E>())) tile_kernel(TileArgs A) {
-    StageRegs<T, KIND, (1 << LOGL), (1 << LOGL) / E> R;  // not used without PIPE
-    tile_body<T, LOGL, E, KIND, false>(A, tile_id(A), ~0u, R);
-}
-
-// Persistent form for passes with staged (column) input or direct complex
-// input: work-group
-// g runs the tiles of indices g, g + G, g + 2G, ... (G = grid size, a
-// multiple of the XCD count, so that the XCD tile order keeps each
-// work-group inside its XCD's tile range) and loads the next tile's input
-// into registers before transforming the current one, so that its memory
-// latency hides behind the FFT even at one work-group per CU.  The launch
-// keeps the grid below the tile count.
-template <typename T, int LOGL, int E, int KIND>
-__global__ void __launch_bounds__((tile_threads<T, E>())) tile_kernel_p(TileArgs A) {
-    constexpr int L = 1 << LOGL, TPT = L / E;
-    StageRegs<T, KIND, L, TPT> R;
-    const unsigned long long G = gridDim.x;
-    unsigned b = blockIdx.x;
-    if (b >= A.ntiles) return;  // uniform over the work-group
-    unsigned tile = tile_of(A, b);
-    const int t = threadIdx.x;
-    if (kind_complex_in(KIND) && (KIND == K_DFT || KIND == K_INVDFT) && A.in_mode == IO_DIRECT) {
-        // the core's lane mapping of tile_body
-        const int c = A.cfast ? (t & (A.C - 1)) : t / TPT;
-        const int tl = A.cfast ? (t >> A.lgC) : t - (t / TPT) * TPT;
-        const TransformRef me = locate(A, tile * (unsigned)A.C + (unsigned)c);
-        if constexpr (KIND == K_DFT || KIND == K_INVDFT) direct_load<T, L, E>(A, me, tl, R.c);
-    } else {
-        // the staging lane mapping of tile_body
-        const bool rowm = (A.in_mode == IO_STAGE_ROW);
-        const int sc_ = rowm ? t / TPT : (t & (A.C - 1));
-        const int stl = rowm ? t - sc_ * TPT : (t >> A.lgC);
-        const TransformRef src = locate(A, tile * (unsigned)A.C + (unsigned)sc_);
-        stage_load<T, KIND, L, TPT>(A, src, stl, R);  // transform 0 when invalid (unused)
-    }
-    for (;;) {
-        const unsigned long long nb = b + G;
-        const unsigned next = nb < A.ntiles ? tile_of(A, (unsigned)nb) : ~0u;
-        tile_body<T, LOGL, E, KIND, true>(A, tile, next, R);
-        if (next == ~0u) break;
-        b = (unsigned)nb;
-        tile = next;
-        __syncthreads();  // every lane is done reading this tile's slots
     }
 }
 

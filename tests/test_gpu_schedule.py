@@ -18,8 +18,7 @@ SPECS = ["", "tpw=128", "tpw=512", "cfb=64", "cfb=256", "xcd=0", "wave=0",
          "fused=0", "falt=1", "wpc=1", "wpc=4", "split=-1", "split=1",
          "split2=0", "split2=0,split=-1", "split2=0,split=1", "split2=1", "split2=2", "split2=3",
          "fused=0,split=-1", "fused=0,split=1",
-         "rsplit=11", "rsplit=12", "rsplit=21", "rsplit=22", "rsplit=31", "rsplit=32", "rsplit=41",
-         "pipe=1", "pipe=1,cfcore=0", "pipe=1,cfb=256", "pipe=1,xcd=0"]
+         "rsplit=11", "rsplit=12", "rsplit=21", "rsplit=22", "rsplit=31", "rsplit=32", "rsplit=41"]
 
 
 class Packed:
@@ -182,47 +181,3 @@ def test_residue_split_every_code(oracle64, prec):
                 e = rel_l2(y, refs[i])
                 assert e <= bar, (code, logn, i, k, e)
         batch.close()
-
-
-@pytest.mark.parametrize("prec", [32, 64])
-def test_pipelined_column_tiles_every_kind(oracle64, prec):
-    """The persistent pipelined tile kernel (tile_kernel_p): staged column
-    input of every kind and direct complex input (rows, c-fast columns, 1D
-    entries at and below the single-tile limit), with more tiles than CUs so
-    that the kernel loops and prefetches; with and without the XCD tile
-    order."""
-    kinds = ["dft", "invdft", "realdft", "invrealdft", "dct2", "dst2", "dct3", "dst3", "dct4", "dst4"]
-    shape = (256, 256)
-    cnt = 40
-    rng = np.random.default_rng(21 + prec)
-    entries, xs = [], []
-    for k in kinds:
-        for _ in range(cnt):
-            entries.append((k, shape))
-            xs.append(make_input(k, shape, prec, int(rng.integers(1 << 30)), oracle64))
-    lim = 1 << (14 if prec == 32 else 13)
-    for k, n in (("dft", lim), ("invdft", lim // 2)):
-        for _ in range(300):
-            entries.append((k, (n,)))
-            xs.append(make_input(k, (n,), prec, int(rng.integers(1 << 30))))
-    pk = Packed(entries, xs, prec)
-    ngroups = pk.batch.describe().count("\n")
-    check = [i * cnt + j for i in range(len(kinds)) for j in (0, cnt // 2, cnt - 1)]
-    base = len(kinds) * cnt
-    check += [base, base + 150, base + 299, base + 300, base + 451, base + 599]
-    refs = {}
-    for i in check:
-        kind, shp = entries[i]
-        x = xs[i].astype(np.complex128 if is_complex_in(kind) else np.float64)
-        refs[i] = oracle64.run(kind, x, shp)
-    for spec in ("pipe=1", "pipe=1,cfcore=0", "pipe=1,cfcore=0,xcd=0", "pipe=1,split2=0"):
-        for g in range(ngroups):
-            pk.batch.schedule(g, spec)
-        print(f"[case] schedule '{spec}' prec={prec}", flush=True)
-        hout = pk.run()
-        for i in check:
-            kind, shp = entries[i]
-            bar = tol64(shp) if prec == 64 else TOL32
-            e = rel_l2(pk.output(hout, i), refs[i])
-            assert e <= bar, (spec, i, kind, e)
-    pk.batch.close()
