@@ -71,8 +71,7 @@ void minfft_free_batch(minfft_batch_plan *p);
 /* Replace group `group`'s schedule (groups are numbered as listed by
    minfft_batch_describe) by the process-wide one with the "key=value,..."
    changes in `spec` (keys tpw, cfb, cfcore, wave, xcd, e32, split, split2,
-   fused, wpc, falt;
-   NULL or "" = the process-wide schedule).  The next run re-captures the
+   fused, wpc, falt, rsplit; NULL or "" = the process-wide schedule).  The next run re-captures the
    graph.  Returns 0, or -1 on an unknown key or group. */
 int minfft_batch_schedule(minfft_batch_plan *p, int group, const char *spec);
 
