@@ -181,3 +181,25 @@ def test_residue_split_every_code(oracle64, prec):
                 e = rel_l2(y, refs[i])
                 assert e <= bar, (code, logn, i, k, e)
         batch.close()
+
+
+def test_autotuner_single_group_plan(oracle64):
+    """A plan with one group is tuned with the residue-split kernels among
+    the candidates (its cache key differs from a multi-group plan's); the
+    chosen schedule is reported and the results agree with the oracle."""
+    n = 1 << 14
+    cnt = 320
+    entries = [("dft", (n,))] * cnt
+    rng = np.random.default_rng(9)
+    xs = [make_input(k, s, 32, int(rng.integers(1 << 30))) for k, s in entries]
+    pk = Packed(entries, xs, 32)
+    desc = pk.batch.describe()
+    print(desc)
+    lines = desc.strip().split("\n")
+    assert len(lines) == 1 and " tuned:" in lines[0] and "rsplit=" in lines[0], desc
+    hout = pk.run()
+    for i in (0, 1, cnt // 2, cnt - 1):
+        kind, shape = entries[i]
+        ref = oracle64.run(kind, xs[i].astype(np.complex128), shape)
+        assert rel_l2(pk.output(hout, i), ref) <= TOL32
+    pk.batch.close()
