@@ -849,6 +849,7 @@ Plan* make_plan(Family fam, Prec prec, int d, const int* Ns) {
         for (int k = 0; k < r.d; ++k) r.n[k] = Ns[k];
         r.count = 1;
         r.tbuf = p->tbuf;
+        r.solo = true;  // a plan's passes run one after another on its stream
         if (d <= 3) {
             const TuneResult t = autotune_group(p->ws, r, p->stream);
             p->sched = t.sched;
