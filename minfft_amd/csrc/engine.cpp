@@ -328,10 +328,11 @@ static int launch_tile(Prec prec, int kind, int logL, const void* tw, const void
         A.cfast = (A.in_mode == IO_STAGE_CF && cf_core) ? 1 : 0;
     }
     // split-FP16 MFMA tiles: the column variant when both sides are columns
-    // of adjacent transforms (direct I/O, 16 transforms per work-group);
-    // otherwise the row variant (two transforms per wavefront, direct I/O
-    // for contiguous rows, other I/O staged)
-    const bool s16cf = split16 && in_cf && out_cf;
+    // of adjacent transforms (direct I/O, 16 transforms per work-group) and
+    // the schedule wants a c-fast core; otherwise the row variant (two
+    // transforms per wavefront, direct I/O for contiguous rows, other I/O
+    // staged through LDS)
+    const bool s16cf = split16 && in_cf && out_cf && cf_core;
     if (s16cf) {
         fn = bigtable().mfma512cf[kind == K_INVDFT];
         A.cfast = 1;

@@ -203,3 +203,38 @@ def test_autotuner_single_group_plan(oracle64):
         ref = oracle64.run(kind, xs[i].astype(np.complex128), shape)
         assert rel_l2(pk.output(hout, i), ref) <= TOL32
     pk.batch.close()
+
+
+def test_split_fp16_schedules(oracle64):
+    """The split-FP16 MFMA tiles under the schedules that change their launch
+    shape: column passes on the column variant (c-fast core) or on the row
+    variant with LDS-staged column input and output (cfcore=0)."""
+    import torch
+    shapes = [(512, 64), (64, 512), (512, 8, 2), (8, 512, 4)]
+    kinds = ["dft", "invdft", "dft", "invdft"]
+    rng = np.random.default_rng(33)
+    xs = [make_input(k, sh, 32, int(rng.integers(1 << 30))) for k, sh in zip(kinds, shapes)]
+    offs, pos = [], 0
+    for x in xs:
+        offs.append(pos)
+        pos += x.nbytes + 64
+    hin = np.zeros(pos, dtype=np.uint8)
+    for x, o in zip(xs, offs):
+        hin[o:o + x.nbytes] = np.frombuffer(x.tobytes(), dtype=np.uint8)
+    din = torch.from_numpy(hin).cuda()
+    dout = torch.zeros(pos, dtype=torch.uint8, device="cuda")
+    batch = mf.Batch([(k, sh, o, o) for k, sh, o in zip(kinds, shapes, offs)], mf.PREC_SPLIT_FP16)
+    refs = [oracle64.run(k, x.astype(np.complex128), sh) for k, x, sh in zip(kinds, xs, shapes)]
+    for spec in ("", "cfcore=0", "cfcore=1", "cfcore=0,cfb=256", "xcd=0"):
+        for g in range(len(shapes)):
+            batch.schedule(g, spec)
+        print(f"[case] split-FP16 schedule '{spec}'", flush=True)
+        dout.zero_()
+        batch.run(din.data_ptr(), dout.data_ptr(), None)
+        torch.cuda.synchronize()
+        hout = dout.cpu().numpy()
+        for i, (x, o) in enumerate(zip(xs, offs)):
+            y = hout[o:o + x.nbytes].view(np.complex64).reshape(shapes[i])
+            e = rel_l2(y, refs[i])
+            assert e <= TOL32, (spec, i, kinds[i], shapes[i], e)
+    batch.close()
