@@ -10,10 +10,11 @@
 // offsets, so a group of any size costs one launch per axis (up to three
 // for an axis longer than one tile: the two steps of the N = N1*N2
 // decomposition plus a real kind's pre or post pass, unless the fused or the
-// radix-2-split kernel covers it in one).  Each group's launch schedule is picked when the
-// plan is made by timing candidates on the device (autotune.cpp).  All
-// groups' launches are captured once into a HIP graph per (in, out, stream)
-// and replayed.
+// radix-2-split kernel covers it in one).  The short 1D complex groups
+// (2^4 .. 2^12 points) of one direction instead share one persistent launch
+// (multi.hpp).  Each group's launch schedule is picked when the plan is made
+// by timing candidates on the device (autotune.cpp).  All launches are
+// captured once into a HIP graph per (in, out, stream) and replayed.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -26,6 +27,7 @@
 #include "autotune.hpp"
 #include "engine.hpp"
 #include "minfft_ext.h"
+#include "multi.hpp"
 
 using namespace mfx;
 
@@ -42,6 +44,17 @@ struct BatchGroup {
     size_t fbytes = 0;     // per-group two-step scratch
     void* tbuf = nullptr;
     TuneResult tune;       // launch schedule of the group's passes
+    bool in_multi = false; // runs inside a persistent short-length launch
+};
+
+// One persistent launch for the short 1D complex groups of one direction
+// (multi.hpp); `args` gets the buffers at enqueue.
+struct MultiLaunch {
+    bool inv = false;
+    std::vector<int> members;  // group indices
+    MultiArgs args;
+    unsigned grid = 0;
+    long long bytes = 0;
 };
 
 struct minfft_batch_plan {
@@ -56,10 +69,21 @@ struct minfft_batch_plan {
     hipStream_t g_st = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    // persistent launches of the short 1D complex groups, and their item
+    // counters (one per launch)
+    std::vector<MultiLaunch> multi;
+    unsigned* d_counter = nullptr;
     // branch streams/events for concurrent groups inside the captured graph
     std::vector<hipStream_t> side;
     std::vector<hipEvent_t> ev;
 };
+
+// short 1D complex groups share one persistent launch per direction
+// (MINFFT_MULTI=0: one launch per group)
+static bool multi_enabled() {
+    const char* e = std::getenv("MINFFT_MULTI");
+    return !(e && std::atoi(e) == 0);
+}
 
 // groups run on up to this many concurrent graph branches
 static int branch_count() {
@@ -75,6 +99,12 @@ static int lg2(long long x) {
     int l = 0;
     while ((1LL << l) < x) ++l;
     return l;
+}
+
+static long long group_bytes(const BatchGroup& g, Prec pr) {
+    long long P = 1;
+    for (int k = 0; k < g.d; ++k) P *= g.n[k];
+    return P * (long long)g.ioff.size() * (long long)real_size(pr);
 }
 
 extern "C" {
@@ -107,6 +137,7 @@ void minfft_free_batch(minfft_batch_plan* p) {
         if (g.d_toff) (void)hipFree(g.d_toff);
         if (g.tbuf) (void)hipFree(g.tbuf);
     }
+    if (p->d_counter) (void)hipFree(p->d_counter);
     free_workspace(p->ws);
     for (auto s : p->side) (void)hipStreamDestroy(s);
     for (auto e : p->ev) (void)hipEventDestroy(e);
@@ -222,7 +253,53 @@ minfft_batch_plan* minfft_mkaux_batch(int count, const minfft_batch_entry* e, in
         p->ws.fs_bytes = fs_max;
     }
     if (hipStreamSynchronize(p->stream) != hipSuccess) return fail("twiddle generation failed");
+    // short 1D complex groups (lengths 2^4 .. 2^12): one persistent launch
+    // per direction, items numbered largest length first
+    const int mgrid = multi_enabled() ? multi_grid(p->prec) : 0;
+    for (int inv = 0; inv < 2 && mgrid > 0; ++inv) {
+        std::vector<int> mem;
+        for (int gi = 0; gi < (int)p->groups.size(); ++gi) {
+            const BatchGroup& g = p->groups[gi];
+            const int lg = lg2(g.n[0]);
+            if (g.d == 1 && g.kind == (inv ? K_INVDFT : K_DFT) && lg >= 4 && lg <= 12 &&
+                (int)mem.size() < kMultiMaxGroups)
+                mem.push_back(gi);
+        }
+        if (mem.empty()) continue;
+        std::stable_sort(mem.begin(), mem.end(),
+                         [&](int a, int b) { return p->groups[a].n[0] > p->groups[b].n[0]; });
+        MultiLaunch ml;
+        ml.inv = inv != 0;
+        ml.members = mem;
+        std::memset(&ml.args, 0, sizeof(ml.args));
+        unsigned long long items = 0;
+        for (int k = 0; k < (int)mem.size(); ++k) {
+            BatchGroup& g = p->groups[mem[k]];
+            g.in_multi = true;
+            const unsigned cnt = (unsigned)g.ioff.size();
+            const unsigned per = (unsigned)(kMultiThreads * 16 / g.n[0]);  // transforms per item
+            MultiGroup& mg = ml.args.g[k];
+            mg.in_off = g.d_ioff;
+            mg.out_off = g.d_ooff;
+            mg.count = cnt;
+            mg.item0 = (unsigned)items;
+            mg.logL = lg2(g.n[0]);
+            items += (cnt + per - 1) / per;
+            ml.bytes += group_bytes(g, p->prec);
+        }
+        if (items > 0x7fffffffull) return fail("batch too large");
+        ml.args.ngroups = (int)mem.size();
+        ml.args.total = (unsigned)items;
+        ml.args.tw = p->ws.tw;
+        ml.grid = (unsigned)std::min<unsigned long long>(items, (unsigned long long)mgrid);
+        p->multi.push_back(ml);
+    }
+    if (!p->multi.empty()) {
+        if (hipMalloc(&p->d_counter, 2 * sizeof(unsigned)) != hipSuccess) return fail("hipMalloc batch counter failed");
+        for (size_t k = 0; k < p->multi.size(); ++k) p->multi[k].args.counter = p->d_counter + k;
+    }
     for (auto& g : p->groups) {
+        if (g.in_multi) continue;  // not launched on its own: nothing to tune
         TuneRequest r;
         r.prec = p->prec;
         r.kind = g.kind;
@@ -239,25 +316,36 @@ minfft_batch_plan* minfft_mkaux_batch(int count, const minfft_batch_entry* e, in
     return p;
 }
 
-static long long group_bytes(const BatchGroup& g, Prec pr) {
-    long long P = 1;
-    for (int k = 0; k < g.d; ++k) P *= g.n[k];
-    return P * (long long)g.ioff.size() * (long long)real_size(pr);
-}
 
 // Enqueue every group.  With branches > 1 the groups are spread (largest
 // first, onto the least-loaded branch) over side streams forked from and
 // joined back into `st`, so that in a captured graph they become parallel
 // branches: short groups fill the tails of the long ones.
 static int enqueue_batch(minfft_batch_plan* p, const void* in, void* out, hipStream_t st, int branches) {
-    const int nb = std::min<int>(branches, (int)p->groups.size());
-    if (nb <= 1) {
-        for (auto& g : p->groups) {
-            TuneScope scope(&g.tune.sched);
-            int rc = run_entries(p->ws, p->prec, g.kind, g.d, g.n, in, out, (unsigned)g.ioff.size(), g.d_ioff,
-                                 g.d_ooff, g.d_foff, g.tbuf, g.d_toff, st);
-            if (rc) return rc;
+    // units of work, largest first: the groups launched on their own and
+    // the persistent short-length launches (index -1 - k)
+    std::vector<std::pair<long long, int>> units;
+    for (int gi = 0; gi < (int)p->groups.size(); ++gi)
+        if (!p->groups[gi].in_multi) units.push_back({group_bytes(p->groups[gi], p->prec), gi});
+    for (int k = 0; k < (int)p->multi.size(); ++k) units.push_back({p->multi[k].bytes, -1 - k});
+    std::stable_sort(units.begin(), units.end(), [](const std::pair<long long, int>& a,
+                                                    const std::pair<long long, int>& b) { return a.first > b.first; });
+    auto run_unit = [&](int u, hipStream_t s) -> int {
+        if (u < 0) {
+            MultiLaunch& ml = p->multi[-1 - u];
+            ml.args.in = in;
+            ml.args.out = out;
+            return launch_multi(p->prec, ml.inv, ml.args, ml.grid, s);
         }
+        BatchGroup& g = p->groups[u];
+        TuneScope scope(&g.tune.sched);
+        return run_entries(p->ws, p->prec, g.kind, g.d, g.n, in, out, (unsigned)g.ioff.size(), g.d_ioff, g.d_ooff,
+                           g.d_foff, g.tbuf, g.d_toff, s);
+    };
+    const int nb = std::min<int>(branches, (int)units.size());
+    if (nb <= 1) {
+        for (const auto& u : units)
+            if (int rc = run_unit(u.second, st)) return rc;
         return 0;
     }
     while ((int)p->side.size() < nb) {
@@ -281,18 +369,18 @@ static int enqueue_batch(minfft_batch_plan* p, const void* in, void* out, hipStr
     for (int b = 0; b < nb; ++b)
         if (hipStreamWaitEvent(p->side[b], p->ev[nb], 0) != hipSuccess) return -1;
     std::vector<long long> load(nb, 0);
-    for (auto& g : p->groups) {  // groups are sorted largest first
+    for (const auto& u : units) {
         int b = (int)(std::min_element(load.begin(), load.end()) - load.begin());
-        // the two-step scratch is shared: groups that use it stay on
-        // branch 0 (1D complex groups whose schedule is single-pass do not)
-        const bool single = g.d == 1 && (g.kind == K_DFT || g.kind == K_INVDFT) &&
-                            c2c_single_pass(p->prec, lg2(g.n[0]), g.tune.sched);
-        if (g.fbytes && !single) b = 0;
-        load[b] += group_bytes(g, p->prec);
-        TuneScope scope(&g.tune.sched);
-        int rc = run_entries(p->ws, p->prec, g.kind, g.d, g.n, in, out, (unsigned)g.ioff.size(), g.d_ioff,
-                             g.d_ooff, g.d_foff, g.tbuf, g.d_toff, p->side[b]);
-        if (rc) return rc;
+        if (u.second >= 0) {
+            // the two-step scratch is shared: groups that use it stay on
+            // branch 0 (1D complex groups whose schedule is single-pass do not)
+            const BatchGroup& g = p->groups[u.second];
+            const bool single = g.d == 1 && (g.kind == K_DFT || g.kind == K_INVDFT) &&
+                                c2c_single_pass(p->prec, lg2(g.n[0]), g.tune.sched);
+            if (g.fbytes && !single) b = 0;
+        }
+        load[b] += u.first;
+        if (int rc = run_unit(u.second, p->side[b])) return rc;
     }
     // join
     for (int b = 0; b < nb; ++b) {
@@ -382,9 +470,14 @@ int minfft_batch_describe(const minfft_batch_plan* p, char* buf, int len) {
         const BatchGroup& g = p->groups[i];
         std::string dims = std::to_string(g.n[0]);
         for (int k = 1; k < g.d; ++k) dims += "x" + std::to_string(g.n[k]);
-        std::snprintf(line, sizeof(line), "group %zu: %s %s count=%zu %s: %s default_us=%.1f tuned_us=%.1f\n", i,
-                      kname[g.kind], dims.c_str(), g.ioff.size(), g.tune.tuned ? "tuned" : "untuned",
-                      describe_tune(g.tune.sched).c_str(), g.tune.us_default, g.tune.us_best);
+        if (g.in_multi)
+            std::snprintf(line, sizeof(line),
+                          "group %zu: %s %s count=%zu persistent: one launch with the other short %s groups\n", i,
+                          kname[g.kind], dims.c_str(), g.ioff.size(), kname[g.kind]);
+        else
+            std::snprintf(line, sizeof(line), "group %zu: %s %s count=%zu %s: %s default_us=%.1f tuned_us=%.1f\n",
+                          i, kname[g.kind], dims.c_str(), g.ioff.size(), g.tune.tuned ? "tuned" : "untuned",
+                          describe_tune(g.tune.sched).c_str(), g.tune.us_default, g.tune.us_best);
         s += line;
     }
     if (buf && len > 0) {

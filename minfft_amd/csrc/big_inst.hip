@@ -2,10 +2,12 @@
 // and their registry (bigtable.hpp): the fused two-step kernel (big.hpp; FP32
 // with 512-thread work-groups, FP64 with 256: two work-groups of ~70 KiB LDS
 // per CU where the registers allow), the radix-2-split single-pass kernel
-// (split2.hpp) and the split-FP16 matrix-core 512-point tile (mfma.hpp).
+// (split2.hpp), the split-FP16 matrix-core 512-point tile (mfma.hpp) and the
+// persistent short-length batch kernel (multi.hpp).
 #include "big.hpp"
 #include "mfma.hpp"
 #include "split2.hpp"
+#include "multi.hpp"
 #include "bigtable.hpp"
 
 namespace mfx {
@@ -39,6 +41,13 @@ static void reg_split2(BigTable& t, int prec) {
 }
 
 void register_big(BigTable& t) {
+    t.multi[0][0] = reinterpret_cast<const void*>(&multi_kernel<double, false>);
+    t.multi[0][1] = reinterpret_cast<const void*>(&multi_kernel<double, true>);
+    t.multi[1][0] = reinterpret_cast<const void*>(&multi_kernel<float, false>);
+    t.multi[1][1] = reinterpret_cast<const void*>(&multi_kernel<float, true>);
+    t.multi_lds[0] = multi_lds_bytes<double>();
+    t.multi_lds[1] = multi_lds_bytes<float>();
+    t.multi_reset = reinterpret_cast<const void*>(&multi_reset_kernel<unsigned>);
     t.mfma512[0] = reinterpret_cast<const void*>(&mfma512_kernel<false>);
     t.mfma512[1] = reinterpret_cast<const void*>(&mfma512_kernel<true>);
     t.mfma512cf[0] = reinterpret_cast<const void*>(&mfma512_cf_kernel<false>);

@@ -1,5 +1,6 @@
 // bigtable.hpp -- registry of the kernels outside the per-kind tile table:
-// fused two-step, radix-2-split, residue-split and split-FP16 matrix-core.
+// fused two-step, radix-2-split, residue-split, split-FP16 matrix-core and
+// the persistent short-length batch kernel.
 #pragma once
 #include <cstddef>
 
@@ -28,6 +29,10 @@ struct BigTable {
     const void* rsplit[2][17][4][2][2];  // [prec][log2 N][log2 P - 1][B - 1][inverse]
     int rsplit_bt[2][17][4][2];
     size_t rsplit_lds[2][17][4][2];
+    // persistent launch for the short 1D complex groups of a batch (multi.hpp)
+    const void* multi[2][2];  // [prec][inverse]
+    size_t multi_lds[2];
+    const void* multi_reset;
 };
 
 void register_big(BigTable& t);

@@ -22,11 +22,13 @@
 #include <map>
 #include <mutex>
 #include <string>
+#include <tuple>
 
 #include "bigtable.hpp"
 #include "ktable.hpp"
 #include "tile.hpp"
 #include "big.hpp"
+#include "multi.hpp"
 #include "autotune.hpp"
 
 namespace mfx {
@@ -214,6 +216,24 @@ TuneScope::~TuneScope() { t_tune = prev; }
 // -------------------------------------------------------------- launching
 static std::mutex g_attr_mu;
 static std::map<std::pair<int, const void*>, size_t> g_attr_lds;  // (device, kernel) -> LDS allowed
+
+// resident work-groups per CU of a kernel at this launch shape (cached)
+static int occupancy(const void* fn, unsigned block, size_t lds) {
+    static std::mutex mu;
+    static std::map<std::tuple<int, const void*, unsigned, size_t>, int> cache;
+    std::lock_guard<std::mutex> lk(mu);
+    const auto key = std::make_tuple(device().id, fn, block, lds);
+    const auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
+    if (lds > 65536) (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, (int)block, lds) != hipSuccess || n < 1) {
+        (void)hipGetLastError();
+        n = 1;
+    }
+    cache[key] = n;
+    return n;
+}
 
 static int launch_fn(const void* fn, unsigned grid, unsigned block, void** args, size_t lds,
                      hipStream_t st) {
@@ -484,6 +504,27 @@ static int launch_rsplit(const Workspace& ws, Prec prec, int kind, int logL, con
     A.xcd = 1;
     void* args[] = {&A};
     return launch_fn(fn, (unsigned)grid, (unsigned)bt.rsplit_bt[pi][logL][lp - 1][b - 1], args, lds, st);
+}
+
+int multi_grid(Prec prec) {
+    if (!ensure_device()) return 0;
+    const BigTable& bt = bigtable();
+    const int pi = prec == PREC_F64 ? 0 : 1;
+    if (!bt.multi[pi][0] || bt.multi_lds[pi] > (size_t)device().lds_max) return 0;
+    const int occ = std::max(occupancy(bt.multi[pi][0], kMultiThreads, bt.multi_lds[pi]),
+                             occupancy(bt.multi[pi][1], kMultiThreads, bt.multi_lds[pi]));
+    return occ * device().ncu;
+}
+
+int launch_multi(Prec prec, bool inv, const MultiArgs& a, unsigned grid, hipStream_t st) {
+    const BigTable& bt = bigtable();
+    const int pi = prec == PREC_F64 ? 0 : 1;
+    unsigned* ctr = a.counter;
+    void* rargs[] = {&ctr};
+    if (launch_fn(bt.multi_reset, 1, 1, rargs, 0, st)) return -1;
+    MultiArgs m = a;
+    void* args[] = {&m};
+    return launch_fn(bt.multi[pi][inv ? 1 : 0], grid, kMultiThreads, args, bt.multi_lds[pi], st);
 }
 
 std::vector<int> rsplit_codes(Prec prec, int logN) {

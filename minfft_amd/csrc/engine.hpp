@@ -149,6 +149,12 @@ int needed_tw_log(int kind, int logN);
 int axis_pass(const Workspace& ws, Prec prec, int kind, int logN,
               const void* in, long long in_e, void* out, long long out_e,
               const Levels& lv, hipStream_t st);
+// persistent launch for the short 1D complex groups of a batch (multi.hpp):
+// resident work-groups on this device (0 = not available), and one launch
+// (counter reset, then the kernel) on st
+struct MultiArgs;
+int multi_grid(Prec prec);
+int launch_multi(Prec prec, bool inv, const MultiArgs& a, unsigned grid, hipStream_t st);
 // schedule codes (Tune::rsplit) of the residue-split kernels instantiated
 // for complex transforms of length 2^logN whose LDS fits this device
 std::vector<int> rsplit_codes(Prec prec, int logN);

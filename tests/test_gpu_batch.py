@@ -212,3 +212,57 @@ def test_split_fp16_mfma_batch(oracle64):
     print("[case] split-FP16 batch", flush=True)
     worst = run_batch(entries, 32, xs, oracle64, mode=mf.PREC_SPLIT_FP16)
     print(f"split-FP16 worst rel-L2 {worst:.2e}")
+
+
+@pytest.mark.parametrize("prec", [32, 64])
+def test_persistent_short_lengths(oracle64, prec, monkeypatch):
+    """Short 1D complex groups (2^4 .. 2^12, both directions) run in one
+    persistent launch per direction (multi.hpp): entries at offsets one
+    complex item off the 16-byte grid, counts that leave the last item of a
+    group partial; the results of the persistent launch and of one launch
+    per group both agree with the oracle (they need not agree bit for bit:
+    the compiler may contract multiply-adds differently in the two
+    kernels)."""
+    torch = _torch()
+    rng = np.random.default_rng(40 + prec)
+    cdt = np.complex128 if prec == 64 else np.complex64
+    cs = np.dtype(cdt).itemsize
+    entries, xs = [], []
+    for lg in range(4, 13):
+        for kind in ("dft", "invdft"):
+            for _ in range(int(rng.integers(1, 90))):
+                n = 1 << lg
+                entries.append((kind, (n,)))
+                xs.append((rng.uniform(-1, 1, n) + 1j * rng.uniform(-1, 1, n)).astype(cdt))
+    order = rng.permutation(len(entries))
+    entries = [entries[i] for i in order]
+    xs = [xs[i] for i in order]
+    offs, pos = [], cs
+    for x in xs:
+        offs.append(pos)
+        pos += x.nbytes + cs
+    hin = np.zeros(pos + cs, dtype=np.uint8)
+    for x, o in zip(xs, offs):
+        hin[o:o + x.nbytes] = np.frombuffer(x.tobytes(), dtype=np.uint8)
+    din = torch.from_numpy(hin).cuda()
+    mode = mf.PREC_FP64 if prec == 64 else mf.PREC_FP32
+    outs = {}
+    for flag in ("1", "0"):
+        monkeypatch.setenv("MINFFT_MULTI", flag)
+        b = mf.Batch([(k, s, o, o) for (k, s), o in zip(entries, offs)], mode)
+        desc = b.describe()
+        assert ("persistent" in desc) == (flag == "1"), desc
+        dout = torch.zeros(pos + cs, dtype=torch.uint8, device="cuda")
+        print(f"[case] persistent short lengths MINFFT_MULTI={flag} prec={prec}", flush=True)
+        for _ in range(2):  # capture, then replay
+            b.run(din.data_ptr(), dout.data_ptr(), None)
+            torch.cuda.synchronize()
+        outs[flag] = dout.cpu().numpy()
+        b.close()
+    for i in range(0, len(entries), 3):
+        kind, shape = entries[i]
+        ref = oracle64.run(kind, xs[i].astype(np.complex128), shape)
+        bar = tol64(shape) if prec == 64 else TOL32
+        for f in ("1", "0"):
+            y = outs[f][offs[i]:offs[i] + xs[i].nbytes].view(cdt)
+            assert rel_l2(y, ref) <= bar, (f, i, kind, shape)

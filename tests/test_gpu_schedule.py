@@ -116,8 +116,8 @@ def test_every_schedule_matches_oracle(oracle64, prec):
 def test_autotuner_tunes_large_groups(oracle64):
     """A group with enough work is timed at plan time; its schedule is
     reported and the results agree with the oracle."""
-    n = 1 << 12
-    cnt = 2048
+    n = 1 << 13
+    cnt = 1024
     entries = [("dft", (n,))] * cnt + [("dct2", (16,))] * 3
     rng = np.random.default_rng(5)
     xs = [make_input(k, s, 32, int(rng.integers(1 << 30))) for k, s in entries]
