@@ -261,8 +261,9 @@ minfft_batch_plan* minfft_mkaux_batch(int count, const minfft_batch_entry* e, in
         for (int gi = 0; gi < (int)p->groups.size(); ++gi) {
             const BatchGroup& g = p->groups[gi];
             const int lg = lg2(g.n[0]);
+            // (split-FP16 plans keep their 512-point groups on the matrix-core tile)
             if (g.d == 1 && g.kind == (inv ? K_INVDFT : K_DFT) && lg >= 4 && lg <= 12 &&
-                (int)mem.size() < kMultiMaxGroups)
+                !(p->ws.split16 && lg == 9) && (int)mem.size() < kMultiMaxGroups)
                 mem.push_back(gi);
         }
         if (mem.empty()) continue;

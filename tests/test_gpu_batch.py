@@ -266,3 +266,15 @@ def test_persistent_short_lengths(oracle64, prec, monkeypatch):
         for f in ("1", "0"):
             y = outs[f][offs[i]:offs[i] + xs[i].nbytes].view(cdt)
             assert rel_l2(y, ref) <= bar, (f, i, kind, shape)
+
+
+def test_split_fp16_keeps_512_on_matrix_cores():
+    """In a split-FP16 plan the 512-point 1D groups keep their launch (the
+    matrix-core tile); other short lengths join the persistent launch."""
+    b = mf.Batch([("dft", (512,), 0, 0), ("dft", (256,), 8192, 8192)], mf.PREC_SPLIT_FP16)
+    lines = b.describe().strip().split("\n")
+    b.close()
+    l512 = [ln for ln in lines if "dft 512 " in ln]
+    l256 = [ln for ln in lines if "dft 256 " in ln]
+    assert l512 and "persistent" not in l512[0], lines
+    assert l256 and "persistent" in l256[0], lines
