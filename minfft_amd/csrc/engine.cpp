@@ -511,7 +511,9 @@ int multi_grid(Prec prec) {
     const BigTable& bt = bigtable();
     const int pi = prec == PREC_F64 ? 0 : 1;
     if (!bt.multi[pi][0] || bt.multi_lds[pi] > (size_t)device().lds_max) return 0;
-    const int occ = std::max(occupancy(bt.multi[pi][0], kMultiThreads, bt.multi_lds[pi]),
+    // resident work-groups of both directions (a larger grid would still be
+    // correct: late work-groups find the items taken)
+    const int occ = std::min(occupancy(bt.multi[pi][0], kMultiThreads, bt.multi_lds[pi]),
                              occupancy(bt.multi[pi][1], kMultiThreads, bt.multi_lds[pi]));
     return occ * device().ncu;
 }
