@@ -51,7 +51,8 @@ def _units():
             u.append((o, "inst.hip", [f"-DINST_F64={f64}", f"-DINST_KIND={kind}"], "engine"))
     u.append((os.path.join(OBJ, "twiddle.o"), "twiddle.hip", [], "engine"))
     u.append((os.path.join(OBJ, "big_inst.o"), "big_inst.hip", [], "engine"))
-    u.append((os.path.join(OBJ, "rsplit_inst.o"), "rsplit_inst.hip", [], "engine"))
+    for part in range(3):
+        u.append((os.path.join(OBJ, f"rsplit_inst{part}.o"), "rsplit_inst.hip", [f"-DRSPLIT_PART={part}"], "engine"))
     u.append((os.path.join(OBJ, "engine.o"), "engine.cpp", ["-x", "hip"], "engine"))
     u.append((os.path.join(OBJ, "batch.o"), "batch.cpp", ["-x", "hip"], "engine"))
     u.append((os.path.join(OBJ, "autotune.o"), "autotune.cpp", ["-x", "hip"], "engine"))

@@ -36,6 +36,13 @@ struct BigTable {
 };
 
 void register_big(BigTable& t);
-void register_rsplit(BigTable& t);
+void register_rsplit_part0(BigTable& t);  // rsplit_inst.hip, one unit per transform length
+void register_rsplit_part1(BigTable& t);
+void register_rsplit_part2(BigTable& t);
+inline void register_rsplit(BigTable& t) {
+    register_rsplit_part0(t);
+    register_rsplit_part1(t);
+    register_rsplit_part2(t);
+}
 
 }  // namespace mfx

@@ -18,28 +18,41 @@ static void reg_rsplit(BigTable& t, int prec) {
     t.rsplit_lds[prec][lg][LOGP - 1][B - 1] = rsplit_lds_bytes<T, LOGM>();
 }
 
-void register_rsplit(BigTable& t) {
-    // FP32: 2^16 as 8 x 2^13 (one or two residues per work-group) or
-    // 4 x 2^14 or 16 x 2^12 (16 blocks with two residues would spill); 2^15 as 4 x 2^13 or 8 x 2^12; 2^14 as
-    // 4 x 2^12 or 2 x 2^13
+// three units (RSPLIT_PART 0, 1, 2: transform lengths 2^14, 2^15, 2^16)
+// so that the instances compile in parallel
+#ifndef RSPLIT_PART
+#error "RSPLIT_PART must be defined"
+#endif
+#if RSPLIT_PART == 0
+void register_rsplit_part0(BigTable& t) {
+    // 2^14: 4 x 2^12 (one or two residues per work-group) or 2 x 2^13
+    reg_rsplit<float, 12, 2, 1, 4>(t, 1);
+    reg_rsplit<float, 12, 2, 2, 4>(t, 1);
+    reg_rsplit<float, 13, 1, 1, 4>(t, 1);
+    reg_rsplit<double, 12, 2, 1, 2>(t, 0);
+    reg_rsplit<double, 12, 2, 2, 2>(t, 0);
+}
+#elif RSPLIT_PART == 1
+void register_rsplit_part1(BigTable& t) {
+    // 2^15: 4 x 2^13 or 8 x 2^12
+    reg_rsplit<float, 13, 2, 1, 4>(t, 1);
+    reg_rsplit<float, 13, 2, 2, 4>(t, 1);
+    reg_rsplit<float, 12, 3, 1, 4>(t, 1);
+    reg_rsplit<float, 12, 3, 2, 4>(t, 1);
+    reg_rsplit<double, 12, 3, 1, 2>(t, 0);
+    reg_rsplit<double, 12, 3, 2, 2>(t, 0);
+}
+#else
+void register_rsplit_part2(BigTable& t) {
+    // 2^16: 8 x 2^13 (one or two residues per work-group), 4 x 2^14 or
+    // 16 x 2^12 (16 blocks with two residues would spill)
     reg_rsplit<float, 13, 3, 1, 4>(t, 1);
     reg_rsplit<float, 13, 3, 2, 4>(t, 1);
     reg_rsplit<float, 14, 2, 1, 4>(t, 1);
     reg_rsplit<float, 14, 2, 2, 4>(t, 1);
     reg_rsplit<float, 12, 4, 1, 4>(t, 1);
-    reg_rsplit<float, 13, 2, 1, 4>(t, 1);
-    reg_rsplit<float, 13, 2, 2, 4>(t, 1);
-    reg_rsplit<float, 12, 3, 1, 4>(t, 1);
-    reg_rsplit<float, 12, 3, 2, 4>(t, 1);
-    reg_rsplit<float, 12, 2, 1, 4>(t, 1);
-    reg_rsplit<float, 12, 2, 2, 4>(t, 1);
-    reg_rsplit<float, 13, 1, 1, 4>(t, 1);
-    // FP64: slots of 2^12 points (two work-groups per CU)
-    reg_rsplit<double, 12, 2, 1, 2>(t, 0);
-    reg_rsplit<double, 12, 2, 2, 2>(t, 0);
-    reg_rsplit<double, 12, 3, 1, 2>(t, 0);
-    reg_rsplit<double, 12, 3, 2, 2>(t, 0);
     reg_rsplit<double, 12, 4, 1, 2>(t, 0);
 }
+#endif
 
 }  // namespace mfx
