@@ -59,3 +59,21 @@ def test_kernels_built_for_gfx950():
     output, read as bytes)."""
     data = open(mf.lib_path("libminfft_engine.so"), "rb").read()
     assert b"amdgcn-amd-amdhsa--gfx950" in data
+
+
+def _build_c_app(tmp_path, single):
+    """A C89 program written against minfft.h (tests/c_app/app.c), compiled
+    unchanged and linked against the drop-in library of its precision."""
+    lib_dir = os.path.dirname(mf.lib_path("libminfft.so"))
+    exe = os.path.join(str(tmp_path), f"app{int(single)}")
+    cmd = ["gcc", "-std=c89", "-pedantic", "-Wall", "-Wextra", "-Werror", "-O2",
+           f"-DMINFFT_SINGLE={int(single)}", "-I", INC, os.path.join(ROOT, "tests", "c_app", "app.c"),
+           "-L", lib_dir, "-lminfft_single" if single else "-lminfft", f"-Wl,-rpath,{lib_dir}", "-lm", "-o", exe]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    return exe
+
+
+@pytest.mark.parametrize("single", [False, True])
+def test_c_application_builds_against_the_dropin(tmp_path, single):
+    assert os.path.exists(_build_c_app(tmp_path, single))

@@ -214,3 +214,17 @@ def test_mismatched_aux_is_reported(libs):
     with pytest.raises(mf.MinfftError):
         libs[32].run_ptr("dft", x.ctypes.data, x.ctypes.data, aux.h)
     aux.close()
+
+
+@pytest.mark.parametrize("single", [False, True])
+def test_c_application_runs_on_the_dropin(tmp_path, single):
+    """The C89 program of tests/c_app (written against minfft.h, host
+    arrays) runs on the GPU through libminfft.so / libminfft_single.so and
+    meets its checks: a direct O(N^2) DFT and the inverse-pair identities
+    of every kind."""
+    import subprocess
+    from tests.test_abi import _build_c_app
+    exe = _build_c_app(tmp_path, single)
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    print(r.stdout)
+    assert r.returncode == 0 and r.stdout.strip().endswith("ok"), (r.stdout, r.stderr)
