@@ -179,6 +179,55 @@ def cpu_baseline(sizes, budget_s=12.0):
                       f"FP32 {'-O3 -march=x86-64-v3' if fast else '-O2'}, one transform at a time"}
 
 
+def cpu_baseline_all_cores(sizes, single, budget_s=4.0, max_threads=16):
+    """Reference minfft (oracle/_ref, FP32) on the host's cores: one thread
+    per core (at most 16, the host share of one GPU), each with its own
+    auxes (a reference aux owns scratch), thread k taking entries k, k+T, ...
+    of the workload, repeated to fill about budget_s of wall time; ctypes
+    releases the GIL inside the calls."""
+    from concurrent.futures import ThreadPoolExecutor
+    from oracle.oracle import Reference, ref_available
+    if not single:
+        return None
+    fast = "-O3" in single["sample"]
+    if not ref_available(32, fast=fast):
+        return None
+    threads = max(1, min(max_threads, os.cpu_count() or 1))
+    step_flops = float(sum(5.0 * int(n) * math.log2(int(n)) for n in sizes))
+    reps = max(1, int(math.ceil(budget_s * threads * single["value"] * 1e9 / step_flops)))
+    ref = Reference(32, fast=fast)
+    f = ref.fn("dft")
+    nmax = int(max(sizes))
+
+    def worker(k):
+        auxes = {}
+        rng = np.random.default_rng(SEED + 2 + k)
+        x = (rng.uniform(-1, 1, nmax) + 1j * rng.uniform(-1, 1, nmax)).astype(np.complex64)
+        y = np.empty_like(x)
+        mine = [int(n) for n in sizes[k::threads]]
+        for n in set(mine):
+            auxes[n] = ref.mkaux("dft", (n,))
+        barrier.wait()
+        for _ in range(reps):
+            for n in mine:
+                f(x.ctypes.data, y.ctypes.data, auxes[n])
+        for a in auxes.values():
+            ref.free(a)
+
+    import threading
+    barrier = threading.Barrier(threads + 1)
+    with ThreadPoolExecutor(max_workers=threads) as ex:
+        futs = [ex.submit(worker, k) for k in range(threads)]
+        barrier.wait()
+        t0 = time.perf_counter()
+        for fu in futs:
+            fu.result()
+        wall = time.perf_counter() - t0
+    return {"value": reps * step_flops / wall / 1e9, "unit": "GFLOP/s", "cores": threads, "kind": "reference",
+            "sample": f"the whole workload {reps} times, entries split over {threads} threads, reference minfft "
+                      f"FP32 {'-O3 -march=x86-64-v3' if fast else '-O2'}, one aux per size per thread"}
+
+
 # --------------------------------------------------------------------------
 # other configurations (printed with --config; the default line is C3)
 # --------------------------------------------------------------------------
@@ -501,6 +550,9 @@ def main():
     cpu = None if a.no_cpu else cpu_baseline(sizes)
     if cpu:
         log(f"[bench] cpu baseline: {cpu}")
+    cpu_all = None if a.no_cpu else cpu_baseline_all_cores(sizes, cpu)
+    if cpu_all:
+        log(f"[bench] cpu baseline, all cores: {cpu_all}")
 
     gflops = flops / (ms32 * 1e-3) / 1e9
     step_achieved = alg_bytes / (ms32 * 1e-3) / 1e9
@@ -535,6 +587,7 @@ def main():
         "fp64_gpu_same_workload": fp64,
         "speedup_vs_fp64_gpu": round(fp64["ms_per_step"] / ms32, 3) if fp64 else None,
         "cpu_baseline": cpu,
+        "cpu_baseline_all_cores": cpu_all,
     }
     print(json.dumps(out), flush=True)
     if a.copy_probe:
