@@ -438,6 +438,22 @@ def config_c4(a, torch, mf):
     a32.close()
     a64.close()
     ms = ms_s
+    # CPU baseline: the reference's own 3D transform (FP32: its aux of
+    # 2^27 points still fits the reference's int sizes) on this input, once
+    cpu = None
+    from oracle.oracle import Reference, ref_available
+    if not a.no_cpu and ref_available(32, fast=True):
+        R = Reference(32, fast=True)
+        xh = x32.cpu().numpy()
+        yh = np.empty_like(xh)
+        ra = R.mkaux("dft", shape)
+        t0 = time.perf_counter()
+        R.fn("dft")(xh.ctypes.data, yh.ctypes.data, ra)
+        dt = time.perf_counter() - t0
+        R.free(ra)
+        del xh, yh
+        cpu = {"value": round(flops / dt / 1e9, 3), "unit": "GFLOP/s", "cores": 1, "kind": "reference",
+               "sample": f"one 512^3 transform of this input, reference minfft FP32 -O3, {dt:.2f} s"}
     return _line("3D C2C 512^3 complex64 GFLOP/s", round(flops / (ms * 1e-3) / 1e9, 3), "GFLOP/s", ms,
                  a.steps, a.warmup, "f32+split-f16",
                  "C4: 512^3 C2C, 64 plane waves + N(0,1e-3) noise; radix-16 stages as split-FP16 MFMA",
@@ -448,7 +464,8 @@ def config_c4(a, torch, mf):
                   "roofline": {"bound": "hbm", "achieved": round(alg / (ms * 1e-3) / 1e9, 1),
                                "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                "frac": round(alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4), "traffic": None,
-                               "kernel": "3 axis passes"}})
+                               "kernel": "3 axis passes"},
+                  "cpu_baseline": cpu})
 
 
 def config_c5(a, torch, mf):
@@ -484,6 +501,24 @@ def config_c5(a, torch, mf):
     N = n * n * count
     alg = 2 * N * 4
     flops = 2.5 * N * 20
+    # CPU baseline: the reference's own 2D transforms (FP32) on 6 images,
+    # two of each type
+    cpu = None
+    from oracle.oracle import Reference, ref_available
+    if not a.no_cpu and ref_available(32, fast=True):
+        R = Reference(32, fast=True)
+        dt = 0.0
+        for i in range(6):
+            xh = x[i].cpu().numpy()
+            yh = np.empty_like(xh)
+            ra = R.mkaux(kinds[i % 3], (n, n))
+            t0 = time.perf_counter()
+            R.fn(kinds[i % 3])(xh.ctypes.data, yh.ctypes.data, ra)
+            dt += time.perf_counter() - t0
+            R.free(ra)
+        cpu = {"value": round(6 * 2.5 * n * n * 20 / dt / 1e9, 3), "unit": "GFLOP/s", "cores": 1,
+               "kind": "reference", "sample": f"6 of the 256 images (2 per type), reference minfft FP32 -O3, "
+                                               f"{dt * 1e3:.0f} ms"}
     return _line("batched 2D DCT-2/DST-2/DCT-4 1024x1024 FP32 GFLOP/s", round(flops / (ms * 1e-3) / 1e9, 3),
                  "GFLOP/s", ms, a.steps, a.warmup, "f32",
                  "C5: 256 x 1024^2 FP32, type = (DCT-2, DST-2, DCT-4)[i mod 3], one batch",
@@ -491,7 +526,8 @@ def config_c5(a, torch, mf):
                   "speedup_vs_fp64_gpu": round(ms64 / ms, 3),
                   "roofline": {"bound": "hbm", "achieved": round(alg / (ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,
                                "unit": "GB/s", "frac": round(alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-                               "traffic": None, "kernel": "2 axis passes per image"}})
+                               "traffic": None, "kernel": "2 axis passes per image"},
+                  "cpu_baseline": cpu})
 
 
 def main():
