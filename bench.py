@@ -116,6 +116,23 @@ def read_traffic(kernel_tag):
     return None
 
 
+def read_step_traffic(alg_bytes):
+    """HBM bytes of one whole step of the default workload: the sum of the
+    per-launch traffic of every kernel of the step (profiles/c3_traffic.txt);
+    None unless the profiled launches' algorithmic bytes add up to the
+    step's (every launch of the step is covered)."""
+    path = os.path.join(ROOT, "profiles", "c3_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    tot, alg = 0.0, 0.0
+    for line in open(path):
+        if line.startswith("traffic "):
+            kv = dict(t.split("=", 1) for t in line.split()[1:] if "=" in t)
+            tot += float(kv["bytes_per_launch"])
+            alg += float(kv["algorithmic_bytes"])
+    return tot if abs(alg - alg_bytes) <= 0.01 * alg_bytes else None
+
+
 def check_sample(xin, xout, sizes, offs, nsample=48):
     """rel-L2 and max-abs error vs the FP64 oracle on a spread of entries."""
     from oracle.oracle import Oracle
@@ -595,6 +612,7 @@ def main():
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
     dom_tag = f"N={dom_n}"
     traffic = read_traffic(dom_tag)
+    step_traffic = read_step_traffic(alg_bytes)
     out = {
         "metric": METRIC,
         "value": round(gflops, 3),
@@ -616,7 +634,9 @@ def main():
                      "kernel": f"dominant length group {dom_tag} x{dom_cnt} (fused two-step launch)",
                      "algorithmic_bytes": dom_bytes, "ms_per_launch": round(dom_ms, 5)},
         "step_roofline": {"achieved": round(step_achieved, 2), "frac": round(step_achieved / HBM_PEAK_GBS, 4),
-                          "algorithmic_bytes": alg_bytes, "what": "whole batch step, all lengths, one graph launch"},
+                          "algorithmic_bytes": alg_bytes, "what": "whole batch step, all lengths, one graph launch",
+                          "traffic": step_traffic,
+                          "traffic_gbs": round(step_traffic / (ms32 * 1e-3) / 1e9, 2) if step_traffic else None},
         "schedule": {"autotuned": os.environ.get("MINFFT_AUTOTUNE", "1") != "0" and "MINFFT_TUNE" not in os.environ,
                      "groups": schedule},
         "accuracy_vs_fp64_oracle": acc,
