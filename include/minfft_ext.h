@@ -50,8 +50,10 @@ typedef struct minfft_batch_entry {
 typedef struct minfft_batch_plan minfft_batch_plan;
 
 /* Build a batch plan: validates entries (NULL on error), groups identical
-   shapes, builds the twiddle tables on the device and picks each group's
-   launch schedule by timing candidates on the device (autotuner; off with
+   shapes, builds the twiddle tables on the device, gathers the short 1D
+   complex groups (2^4..2^12 points) of each direction into one persistent
+   launch (off with MINFFT_MULTI=0) and picks each other group's launch
+   schedule by timing candidates on the device (autotuner; off with
    MINFFT_AUTOTUNE=0; results cached per shape, and in the file named by
    MINFFT_TUNE_CACHE if set). */
 minfft_batch_plan *minfft_mkaux_batch(int count, const minfft_batch_entry *entries, int prec);
@@ -72,12 +74,15 @@ void minfft_free_batch(minfft_batch_plan *p);
    minfft_batch_describe) by the process-wide one with the "key=value,..."
    changes in `spec` (keys tpw, cfb, cfcore, wave, xcd, e32, split, split2,
    fused, wpc, falt, rsplit; NULL or "" = the process-wide schedule).  The next run re-captures the
-   graph.  Returns 0, or -1 on an unknown key or group. */
+   graph.  A group that runs in the batch's persistent launch of short 1D
+   complex groups (described as "persistent") keeps the schedule unused.
+   Returns 0, or -1 on an unknown key or group. */
 int minfft_batch_schedule(minfft_batch_plan *p, int group, const char *spec);
 
 /* Text description of a plan's groups and their chosen schedules (one line
-   per group) into buf (at most len bytes, NUL-terminated).  Returns the
-   length needed including the NUL, or -1. */
+   per group; "persistent" for the short 1D complex groups that share one
+   persistent launch per direction) into buf (at most len bytes,
+   NUL-terminated).  Returns the length needed including the NUL, or -1. */
 int minfft_batch_describe(const minfft_batch_plan *p, char *buf, int len);
 
 /* Enqueue one transform of an aux on device pointers and a caller's stream
