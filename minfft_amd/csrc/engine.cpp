@@ -803,6 +803,7 @@ void free_plan(Plan* p) {
     free_workspace(p->ws);
     if (p->tbuf) (void)hipFree(p->tbuf);
     if (p->stage) (void)hipFree(p->stage);
+    if (p->hstage) (void)hipHostFree(p->hstage);
     p->magic = 0;
     delete p;
 }
@@ -1025,6 +1026,12 @@ int enqueue(Plan& p, int kind, const void* in, void* out, hipStream_t st) {
                        nullptr, st);
 }
 
+// largest host array staged through the aux's pinned bounce buffer
+constexpr size_t kPinnedStageMax = (size_t)64 << 20;
+// largest 1D transform whose kernel works on the pinned buffer directly
+// (one tile pass for every kind at either precision)
+constexpr long long kZeroCopyMaxPoints = 4096;
+
 int execute(Plan& p, int kind, const void* in, void* out) {
     clear_error();
     if (family_of(kind) != p.fam) {
@@ -1034,7 +1041,8 @@ int execute(Plan& p, int kind, const void* in, void* out) {
     int cur = -1;
     if (hipGetDevice(&cur) != hipSuccess || cur != p.dev) (void)hipSetDevice(p.dev);
     const size_t ib = in_bytes(p, kind), ob = out_bytes(p, kind);
-    if (is_device_pointer(in) && is_device_pointer(out)) {
+    const bool in_host = !is_device_pointer(in), out_host = !is_device_pointer(out);
+    if (!in_host && !out_host) {
         int rc = enqueue(p, kind, in, out, p.stream);
         HIPCHK(hipStreamSynchronize(p.stream), "transform");
         return rc;
@@ -1047,11 +1055,49 @@ int execute(Plan& p, int kind, const void* in, void* out) {
         HIPCHK(hipMalloc(&p.stage, need), "hipMalloc staging buffer");
         p.stage_bytes = need;
     }
-    HIPCHK(hipMemcpyAsync(p.stage, in, ib, hipMemcpyDefault, p.stream), "copy in");
+    // Host arrays up to kPinnedStageMax go through a pinned bounce buffer
+    // owned by the aux (a host memcpy plus one DMA each way); larger ones,
+    // or when pinned memory is short, are copied from pageable memory
+    void* hb = nullptr;
+    if (need <= kPinnedStageMax) {
+        if (p.hstage_bytes < need) {
+            if (p.hstage) (void)hipHostFree(p.hstage);
+            p.hstage = nullptr;
+            p.hstage_bytes = 0;
+            if (hipHostMalloc(&p.hstage, need, hipHostMallocDefault) == hipSuccess)
+                p.hstage_bytes = need;
+            else
+                (void)hipGetLastError();
+        }
+        hb = p.hstage;
+    }
+    if (in_host && out_host && hb && p.d == 1 && p.P <= kZeroCopyMaxPoints) {
+        // short 1D transforms run as one tile pass: the kernel reads and
+        // writes the pinned buffer itself (no DMA copies; in place is safe,
+        // as every transform's reads precede its writes in one work-group)
+        std::memcpy(hb, in, ib);
+        int rc = enqueue(p, kind, hb, hb, p.stream);
+        HIPCHK(hipStreamSynchronize(p.stream), "transform");
+        if (rc) return rc;
+        std::memcpy(out, hb, ob);
+        return 0;
+    }
+    if (in_host && hb) {
+        std::memcpy(hb, in, ib);
+        HIPCHK(hipMemcpyAsync(p.stage, hb, ib, hipMemcpyHostToDevice, p.stream), "copy in");
+    } else {
+        HIPCHK(hipMemcpyAsync(p.stage, in, ib, hipMemcpyDefault, p.stream), "copy in");
+    }
     int rc = enqueue(p, kind, p.stage, p.stage, p.stream);
     if (rc) {
         (void)hipStreamSynchronize(p.stream);
         return rc;
+    }
+    if (out_host && hb) {
+        HIPCHK(hipMemcpyAsync(hb, p.stage, ob, hipMemcpyDeviceToHost, p.stream), "copy out");
+        HIPCHK(hipStreamSynchronize(p.stream), "transform");
+        std::memcpy(out, hb, ob);
+        return 0;
     }
     HIPCHK(hipMemcpyAsync(out, p.stage, ob, hipMemcpyDefault, p.stream), "copy out");
     HIPCHK(hipStreamSynchronize(p.stream), "transform");

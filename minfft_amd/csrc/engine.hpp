@@ -97,6 +97,8 @@ struct Plan {
     size_t tbuf_bytes = 0;
     void* stage = nullptr;    // staging buffer for host pointers (lazy)
     size_t stage_bytes = 0;
+    void* hstage = nullptr;   // pinned host bounce buffer for host pointers (lazy,
+    size_t hstage_bytes = 0;  // up to kPinnedStageMax bytes)
     hipStream_t stream = nullptr;
     bool own_stream = true;
     Tune sched;               // launch schedule (autotuned for large plans)

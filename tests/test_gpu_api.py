@@ -228,3 +228,40 @@ def test_c_application_runs_on_the_dropin(tmp_path, single):
     r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
     print(r.stdout)
     assert r.returncode == 0 and r.stdout.strip().endswith("ok"), (r.stdout, r.stderr)
+
+
+def test_mixed_host_and_device_pointers(libs, oracle64):
+    """One side a host array, the other a device array (staged through the
+    aux's pinned bounce buffer on the host side); in place on a host array;
+    and a host array larger than the bounce buffer's 64 MiB limit."""
+    import torch
+    dev = torch.device("cuda:0")
+    print("[case] mixed host/device pointers", flush=True)
+    for prec in (64, 32):
+        cdt = np.complex128 if prec == 64 else np.complex64
+        shape = (64, 128)
+        x = make_input("dft", shape, prec, 17)
+        aux = mf.Aux(libs[prec], "dft", shape)
+        # host in, device out
+        yt = torch.empty(shape, dtype=torch.complex128 if prec == 64 else torch.complex64, device=dev)
+        aux("dft", x, yt)
+        check("dft", shape, prec, yt.cpu().numpy(), x, oracle64)
+        # device in, host out
+        y = np.empty(shape, dtype=cdt)
+        aux("dft", torch.from_numpy(x).to(dev), y)
+        check("dft", shape, prec, y, x, oracle64)
+        # in place on a host array
+        z = x.copy()
+        aux("dft", z, z)
+        check("dft", shape, prec, z, x, oracle64)
+        aux.close()
+    # 2^23 complex128 points = 128 MiB: pageable copies
+    shape = (1 << 23,)
+    x = make_input("dft", shape, 64, 19)
+    y = np.empty_like(x)
+    aux = mf.Aux(libs[64], "dft", shape)
+    aux("dft", x, y)
+    back = np.empty_like(x)
+    aux("invdft", y, back)
+    aux.close()
+    assert rel_l2(back / shape[0], x) <= tol64(shape)
