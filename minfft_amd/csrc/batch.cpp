@@ -73,6 +73,10 @@ struct minfft_batch_plan {
     // counters (one per launch)
     std::vector<MultiLaunch> multi;
     unsigned* d_counter = nullptr;
+    // device staging of minfft_batch's host arrays (grown on demand)
+    void* st_in = nullptr;
+    void* st_out = nullptr;
+    size_t st_in_bytes = 0, st_out_bytes = 0;
     // branch streams/events for concurrent groups inside the captured graph
     std::vector<hipStream_t> side;
     std::vector<hipEvent_t> ev;
@@ -138,6 +142,8 @@ void minfft_free_batch(minfft_batch_plan* p) {
         if (g.tbuf) (void)hipFree(g.tbuf);
     }
     if (p->d_counter) (void)hipFree(p->d_counter);
+    if (p->st_in) (void)hipFree(p->st_in);
+    if (p->st_out) (void)hipFree(p->st_out);
     free_workspace(p->ws);
     for (auto s : p->side) (void)hipStreamDestroy(s);
     for (auto e : p->ev) (void)hipEventDestroy(e);
@@ -504,14 +510,23 @@ int minfft_batch(minfft_batch_plan* p, const void* in, size_t in_bytes, void* ou
         }
         return 0;
     }
-    void* di = nullptr;
-    void* dout_p = nullptr;
-    if (hipMalloc(&di, std::max<size_t>(in_bytes, 1)) != hipSuccess ||
-        hipMalloc(&dout_p, std::max<size_t>(out_bytes, 1)) != hipSuccess) {
-        if (di) (void)hipFree(di);
+    // device staging buffers held by the plan, grown on demand
+    auto grow = [](void*& buf, size_t& have, size_t need) -> bool {
+        need = std::max<size_t>(need, 1);
+        if (have >= need) return true;
+        if (buf) (void)hipFree(buf);
+        buf = nullptr;
+        have = 0;
+        if (hipMalloc(&buf, need) != hipSuccess) return false;
+        have = need;
+        return true;
+    };
+    if (!grow(p->st_in, p->st_in_bytes, in_bytes) || !grow(p->st_out, p->st_out_bytes, out_bytes)) {
         set_error("hipMalloc staging failed");
         return -1;
     }
+    void* di = p->st_in;
+    void* dout_p = p->st_out;
     int rc = 0;
     if (hipMemcpyAsync(di, in, in_bytes, hipMemcpyDefault, p->stream) != hipSuccess) rc = -1;
     // regions of `out` not covered by any entry keep their contents
@@ -519,8 +534,6 @@ int minfft_batch(minfft_batch_plan* p, const void* in, size_t in_bytes, void* ou
     if (!rc) rc = enqueue_batch(p, di, dout_p, p->stream, 1);
     if (!rc && hipMemcpyAsync(out, dout_p, out_bytes, hipMemcpyDefault, p->stream) != hipSuccess) rc = -1;
     if (hipStreamSynchronize(p->stream) != hipSuccess) rc = -1;
-    (void)hipFree(di);
-    (void)hipFree(dout_p);
     if (rc && !*last_error()) set_error("batch execution failed");
     return rc;
 }

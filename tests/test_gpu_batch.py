@@ -179,10 +179,17 @@ def test_batch_host_pointers(oracle64):
     y = np.zeros_like(x)
     b = mf.Batch([("dft", (n,), int(o) * esz, int(o) * esz) for n, o in zip(sizes, offs)], mf.PREC_FP32)
     b.run_sync(x, y)
+    # again, into a larger output whose tail no entry covers (kept as is):
+    # the plan's staging buffers grow and are reused
+    y2 = np.full(len(y) + 100, 7 + 7j, dtype=y.dtype)
+    b.run_sync(x, y2)
+    b.run_sync(x, y2)
     b.close()
     for n, o in zip(sizes, offs):
         ref = oracle64.run("dft", x[o:o + n].astype(np.complex128), (n,))
         assert rel_l2(y[o:o + n], ref) <= TOL32
+        assert rel_l2(y2[o:o + n], ref) <= TOL32
+    assert np.all(y2[len(y):] == 7 + 7j)
 
 
 def test_batch_rejects_bad_entries():
