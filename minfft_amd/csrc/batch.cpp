@@ -167,6 +167,7 @@ minfft_batch_plan* minfft_mkaux_batch(int count, const minfft_batch_entry* e, in
     p->prec_req = prec;
     auto fail = [&](const char* msg) -> minfft_batch_plan* {
         if (msg) set_error(msg);
+        (void)hipGetLastError();  // the failed call's error stays with this plan
         minfft_free_batch(p);
         return nullptr;
     };

@@ -64,6 +64,7 @@ __global__ void twiddle_kernel(double2* __restrict__ tw64, float2* __restrict__ 
 hipError_t launch_twiddles(double2* tw64, float2* tw32, long long total, hipStream_t st) {
     const int bs = 256;
     long long grid = (total + bs - 1) / bs;
+    (void)hipGetLastError();  // a failed call before this one is not this launch's error
     hipLaunchKernelGGL(twiddle_kernel, dim3((unsigned)grid), dim3(bs), 0, st, tw64, tw32, total);
     return hipGetLastError();
 }

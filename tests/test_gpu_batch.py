@@ -285,3 +285,19 @@ def test_split_fp16_keeps_512_on_matrix_cores():
     l256 = [ln for ln in lines if "dft 256 " in ln]
     assert l512 and "persistent" not in l512[0], lines
     assert l256 and "persistent" in l256[0], lines
+
+
+def test_batch_allocation_failure_returns_null_without_leak():
+    """A plan whose scratch exceeds the device memory (3000 inverse real DFTs
+    of 4096^2 FP64 need ~400 GB of complex staging) is refused: the maker
+    returns NULL with a reason, and frees what it had allocated."""
+    import torch
+    torch.cuda.synchronize()
+    free0, total = torch.cuda.mem_get_info()
+    need = 3000 * 4096 * 2049 * 16
+    assert need > total
+    with pytest.raises(mf.MinfftError, match="hipMalloc"):
+        mf.Batch([("invrealdft", (4096, 4096), 0, 0)] * 3000, mf.PREC_FP64)
+    torch.cuda.synchronize()
+    free1, _ = torch.cuda.mem_get_info()
+    assert free1 >= free0 - (64 << 20), (free0, free1)
