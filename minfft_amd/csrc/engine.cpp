@@ -284,7 +284,7 @@ static void fill_levels(TileArgs& A, const Levels& lv, int* fs_level = nullptr) 
 static int launch_tile(Prec prec, int kind, int logL, const void* tw, const void* tw64, const void* in,
                        long long in_e,
                        void* out, long long out_e, const Levels& lv, int fs_level, int fs_logN,
-                       hipStream_t st, bool split16 = false) {
+                       hipStream_t st, bool split16 = false, bool pack = false) {
     const KTable& kt = ktable(prec);
     Device& D = device();
     if (logL > D.maxlog[prec]) {
@@ -306,7 +306,7 @@ static int launch_tile(Prec prec, int kind, int logL, const void* tw, const void
     A.in_e = in_e;
     A.out_e = out_e;
     fill_levels(A, lv, &fs_level);
-    const unsigned long long ntr = lv.count();
+    unsigned long long ntr = lv.count();
     if (ntr == 0) return 0;
     if (ntr > 0xffffffffull) {
         set_error("batch too large");
@@ -363,6 +363,20 @@ static int launch_tile(Prec prec, int kind, int logL, const void* tw, const void
         A.cfast = 0;
         A.in_mode = in_row ? IO_DIRECT : (in_cf ? IO_STAGE_CF : IO_STAGE_ROW);
         A.out_mode = out_row ? IO_DIRECT : (out_cf ? IO_STAGE_CF : IO_STAGE_ROW);
+    }
+    // packed real pair (the DC and Nyquist columns of a real DFT's half
+    // spectrum as one complex column): the c-fast direct form only; the pass
+    // then covers h = 0..H-2
+    if (pack && kind == K_DFT && !split16 && !e32 && fs_logN == 0 && kt.tilepk[logL] && A.cfast &&
+        A.in_mode == IO_DIRECT && A.out_mode == IO_DIRECT && f == 3 && A.B[3] >= 2 && A.in_s[3] == 1 &&
+        A.out_s[3] == 1) {
+        fn = kt.tilepk[logL];
+        A.pack_in = (long long)(A.B[3] - 1);
+        A.pack_out = (long long)(A.B[3] - 1);
+        ntr = ntr / A.B[3] * (A.B[3] - 1);
+        A.B[3] -= 1;
+        A.lgB[3] = (A.B[3] & (A.B[3] - 1)) == 0 ? ilog2i((int)A.B[3]) : -1;
+        A.ntrans = (unsigned)ntr;
     }
     const bool wants_cf = A.cfast || A.in_mode == IO_STAGE_CF || A.out_mode == IO_STAGE_CF;
     const size_t cs = prec == PREC_F64 ? 16 : 8;
@@ -572,7 +586,8 @@ static int c2c_axis(const Workspace& ws, Prec prec, int kind, int logL, const vo
         if (rc <= 0) return rc;
     }
     if (logL <= maxlog)
-        return launch_tile(prec, kind, logL, ws.tw, ws.tw64, in, in_e, out, out_e, lv, -1, 0, st, ws.split16 != 0);
+        return launch_tile(prec, kind, logL, ws.tw, ws.tw64, in, in_e, out, out_e, lv, -1, 0, st, ws.split16 != 0,
+                           lv.pack);
     if (in_e != out_e) {
         set_error("internal: two-step pass needs equal strides");
         return -1;
@@ -988,7 +1003,10 @@ int run_entries(const Workspace& ws, Prec pr, int kind, int d, const int* Ns, co
             Levels a2 = axis_levels(Hm, a, S);
             const bool first = (a == d - 2);
             const void* src = first ? (const void*)dst1 : (const void*)out;
-            rc = axis_pass(ws, pr, K_DFT, ilog2i(Dm[a]), src, S, out, S, mk(a2, first ? d1off : ooff, ooff), st);
+            Levels lv = mk(a2, first ? d1off : ooff, ooff);
+            // the first complex pass sees the real DC and Nyquist columns
+            lv.pack = first && H >= 2;
+            rc = axis_pass(ws, pr, K_DFT, ilog2i(Dm[a]), src, S, out, S, lv, st);
             if (rc) return rc;
         }
         return 0;

@@ -30,6 +30,8 @@ static void reg_tiles(KTable& t) {
         constexpr int E = default_E(LOGL);
         t.tile[INST_KIND][LOGL] = reinterpret_cast<const void*>(&tile_kernel<IT, LOGL, E, INST_KIND>);
         t.tile_E[LOGL] = E;
+        if constexpr (INST_KIND == K_DFT)
+            t.tilepk[LOGL] = reinterpret_cast<const void*>(&tile_kernel<IT, LOGL, E, K_DFT, true>);
         reg_tiles<LOGL + 1>(t);
     }
 }

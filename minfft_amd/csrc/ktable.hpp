@@ -13,6 +13,8 @@ struct KTable {
     // radix-32 variants (32 points per thread, 512-lane work-groups at most):
     // FP32, logL >= 9 only; null elsewhere
     const void* tile32[K_COUNT][kMaxLog + 1];
+    // forward complex tiles with the packed real pair (tile_kernel PACK)
+    const void* tilepk[kMaxLog + 1];
     int maxlog;                               // largest logL instantiated
     const void* pre[K_COUNT];
     const void* post[K_COUNT];

@@ -46,6 +46,8 @@ struct TileArgs {
     int xcd;                           // number of XCDs for the tile swizzle (1 = off)
     int wave;                          // 1: row-mapped core, so a transform's lanes
                                        // share one wavefront when TPT <= 64
+    long long pack_in, pack_out;       // packed-pair tiles: item offset of column
+                                       // H-1 from column 0 (tile_kernel PACK)
 };
 
 struct TransformRef {
@@ -550,7 +552,9 @@ template <typename T> constexpr int max_threads() { return sizeof(T) == 8 ? 512 
 // radix-32 tiles (E = 32) hold twice the points per lane: at most 512 lanes
 template <typename T, int E> constexpr int tile_threads() { return E >= 32 ? 512 : max_threads<T>(); }
 
-template <typename T, int LOGL, int E, int KIND>
+// PACK (forward complex pass over a real DFT's half spectrum, direct c-fast
+// I/O on both sides): see `packed` below.
+template <typename T, int LOGL, int E, int KIND, bool PACK = false>
 __global__ void __launch_bounds__((tile_threads<T, E>())) tile_kernel(TileArgs A) {
     using C2 = c2<T>;
     constexpr int L = 1 << LOGL;
@@ -581,6 +585,13 @@ __global__ void __launch_bounds__((tile_threads<T, E>())) tile_kernel(TileArgs A
     const bool wv_out = wv && A.out_mode != IO_STAGE_CF;
 
     C2 v[E];
+    // Packed real pair: after a real DFT along the last axis, the columns
+    // h = 0 and h = H-1 of the half spectrum are real (the DC and Nyquist
+    // terms of every row), so the transform at h = 0 takes column H-1 as its
+    // imaginary part, z = a + i b, and the pass skips h = H-1.  With
+    // Z = DFT(z): A[k] = (Z[k] + conj Z[-k]) / 2, B[k] = (Z[k] - conj Z[-k]) / 2i.
+    static_assert(!PACK || KIND == K_DFT, "packed pairs: forward complex passes");
+    const bool packed = PACK && me.valid && me.i3 == 0u;
 
     // ---------------- input side ----------------
     // (direct input only for the complex kinds: the real kinds always stage,
@@ -590,7 +601,14 @@ __global__ void __launch_bounds__((tile_threads<T, E>())) tile_kernel(TileArgs A
         // complex input read straight into the owned positions
         const C2* __restrict__ in = reinterpret_cast<const C2*>(A.in) + me.ib;
         if (me.valid) {
-            if (A.in_e == 1) {
+            if (packed) {
+                // real parts of the two columns (reals at twice the item stride)
+                const T* __restrict__ ra = reinterpret_cast<const T*>(in + (long long)tl * A.in_e);
+                const T* __restrict__ rb = reinterpret_cast<const T*>(in + A.pack_in + (long long)tl * A.in_e);
+                const long long step = 2 * A.in_e * TPT;
+#pragma unroll
+                for (int m = 0; m < E; ++m) v[m] = mk<T>(ra[m * step], rb[m * step]);
+            } else if (A.in_e == 1) {
 #pragma unroll
                 for (int m = 0; m < E; ++m) v[m] = in[tl + m * TPT];
             } else {
@@ -654,6 +672,29 @@ __global__ void __launch_bounds__((tile_threads<T, E>())) tile_kernel(TileArgs A
     if constexpr (KIND == K_DFT || KIND == K_INVDFT) {
         if (A.out_mode == IO_DIRECT) {
             C2* __restrict__ out = reinterpret_cast<C2*>(A.out) + me.ob;
+            if constexpr (PACK) {
+                {  // every lane reaches the barrier
+                    // Z through the slot (free: the core ends with a barrier
+                    // after its last slot read) to pair Z[k] with Z[-k]
+                    if (packed) Core<T, LOGL, E, INV>::write_owned(v, slot, tl);
+                    __syncthreads();
+                    if (packed) {
+                        const long long es = A.out_e;
+                        C2* __restrict__ oa = out + (long long)tl * es;
+                        C2* __restrict__ ob = out + A.pack_out + (long long)tl * es;
+                        const long long step = es * TPT;
+                        const T h = (T)0.5;
+#pragma unroll
+                        for (int m = 0; m < E; ++m) {
+                            const int p = tl + m * TPT;
+                            const C2 z = v[m], w = slot[lpad((L - p) & (L - 1))];
+                            oa[m * step] = mk<T>((z.x + w.x) * h, (z.y - w.y) * h);
+                            ob[m * step] = mk<T>((z.y + w.y) * h, (w.x - z.x) * h);
+                        }
+                        return;
+                    }
+                }
+            }
             if (me.valid) {
                 if (A.out_e == 1) {
 #pragma unroll

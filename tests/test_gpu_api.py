@@ -106,6 +106,29 @@ def test_in_place(libs, oracle64, prec, shape):
         check(kind, shape, prec, y, x, oracle64)
 
 
+@pytest.mark.parametrize("prec", [64, 32])
+@pytest.mark.parametrize("shape", [(2, 2), (1, 16), (4096, 4), (16384, 2), (8, 4, 2), (2, 1, 8), (2048, 2048)],
+                         ids=lambda s: "x".join(map(str, s)))
+def test_realdft_packed_dc_nyquist(libs, oracle64, prec, shape):
+    """The first complex pass of a multi-dimensional real DFT transforms the
+    real DC and Nyquist columns as one complex column and separates them
+    after: both columns and the rest against the oracle (column lengths 1 to
+    2^14; FP64 2^14 takes the two-step path, which does not pack)."""
+    print(f"[case] realdft packed pair prec={prec} shape={shape}", flush=True)
+    x = make_input("realdft", shape, prec, 700 + len(shape))
+    y = gpu_run(libs[prec], "realdft", x, shape)
+    yo = oracle64.run("realdft", x.astype(np.float64), shape)
+    bar = tol64(shape) if prec == 64 else TOL32
+    for name, sl in (("all", np.s_[...]), ("DC", np.s_[..., 0]), ("Nyquist", np.s_[..., -1])):
+        e = rel_l2(y[sl], yo[sl])
+        assert e <= bar, f"{name} column(s) {shape} prec={prec}: rel-L2 {e:.3e} > {bar:.1e}"
+    # the DC and Nyquist columns of a real input's spectrum are Hermitian along the other axes
+    for c in (0, -1):
+        col = y[..., c]
+        flip = np.conj(np.roll(np.flip(col, axis=tuple(range(col.ndim))), 1, axis=tuple(range(col.ndim))))
+        assert rel_l2(col, flip) <= 10 * bar
+
+
 @pytest.mark.parametrize("prec,N", [(64, 16384), (64, 32768), (64, 131072), (32, 32768), (32, 65536),
                                     (32, 262144)])
 def test_long_1d_two_step(libs, oracle64, prec, N):
