@@ -288,3 +288,59 @@ def test_mixed_host_and_device_pointers(libs, oracle64):
     aux("invdft", y, back)
     aux.close()
     assert rel_l2(back / shape[0], x) <= tol64(shape)
+
+
+def test_threads_with_distinct_aux(libs, oracle64):
+    """The reference is re-entrant across distinct aux objects: eight host
+    threads make their own aux (one large enough to be tuned when made),
+    run 10 transforms each through host and device pointers at once, and
+    every result matches the oracle."""
+    import threading
+    import torch
+    print("[case] threads with distinct aux", flush=True)
+    cases = [("dft", (1024,), 64), ("invdft", (64, 64), 32), ("realdft", (256, 128), 64),
+             ("dct2", (32, 32), 32), ("dst4", (4096,), 64), ("dft", (8, 16, 32), 32),
+             ("dct3", (128, 64), 64), ("realdft", (1024, 1024), 64)]
+    want, inputs = [], []
+    for i, (kind, shape, prec) in enumerate(cases):
+        x = make_input(kind, shape, prec, 800 + i)
+        inputs.append(x)
+        want.append(oracle64.run(kind, x.astype(np.complex128 if is_complex_in(kind) else np.float64), shape))
+    errors = [None] * len(cases)
+    start = threading.Barrier(len(cases))
+
+    def work(i):
+        kind, shape, prec = cases[i]
+        try:
+            aux = mf.Aux(libs[prec], mf.FAMILY[kind], shape)
+            start.wait(timeout=60)
+            x = inputs[i]
+            odt = (np.complex128 if prec == 64 else np.complex64) if is_complex_out(kind) else \
+                (np.float64 if prec == 64 else np.float32)
+            xd = torch.from_numpy(np.ascontiguousarray(x)).cuda()
+            for r in range(10):
+                if r % 2:
+                    yd = torch.empty(out_shape(kind, shape), dtype=torch.from_numpy(np.empty(0, odt)).dtype,
+                                     device="cuda")
+                    aux(kind, xd, yd)
+                    y = yd.cpu().numpy()
+                else:
+                    y = np.empty(out_shape(kind, shape), dtype=odt)
+                    aux(kind, x, y)
+                bar = tol64(shape) if prec == 64 else TOL32
+                e = rel_l2(y, want[i])
+                if e > bar:
+                    raise AssertionError(f"{kind} {shape} prec={prec} repeat {r}: rel-L2 {e:.3e}")
+            aux.close()
+        except BaseException as exc:  # reported by the main thread
+            errors[i] = exc
+            start.abort()  # no thread keeps waiting for this one
+
+    threads = [threading.Thread(target=work, args=(i,)) for i in range(len(cases))]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join(timeout=100)
+        assert not t.is_alive(), "a worker thread did not finish"
+    for i, e in enumerate(errors):
+        assert e is None, f"case {cases[i]}: {e!r}"
