@@ -59,9 +59,13 @@ typedef struct minfft_batch_plan minfft_batch_plan;
 minfft_batch_plan *minfft_mkaux_batch(int count, const minfft_batch_entry *entries, int prec);
 
 /* Run the batch on device buffers `in` / `out` (device pointers) on the HIP
-   stream `stream` (hipStream_t; NULL = the plan's own stream).  The first run
-   with a given (in, out, stream) captures a HIP graph; later runs replay it.
-   Asynchronous: returns 0 once enqueued, -1 on error. */
+   stream `stream` (hipStream_t; NULL = the plan's own stream).  A plan with
+   several units of work (groups launched on their own, persistent launches)
+   captures a HIP graph on the first run with a given (in, out, stream) and
+   replays it later, its units on concurrent branches; a plan with one unit
+   launches its kernels straight into the stream (MINFFT_GRAPH=1 / 0 forces
+   the graph / direct launches).  Asynchronous: returns 0 once enqueued, -1
+   on error. */
 int minfft_batch_run(minfft_batch_plan *p, const void *in, void *out, void *stream);
 
 /* Same, but synchronous and accepting host or device pointers; in_bytes and

@@ -301,3 +301,57 @@ def test_batch_allocation_failure_returns_null_without_leak():
     torch.cuda.synchronize()
     free1, _ = torch.cuda.mem_get_info()
     assert free1 >= free0 - (64 << 20), (free0, free1)
+
+
+@pytest.mark.parametrize("single_unit", [False, True], ids=["several-units", "one-unit"])
+def test_graph_replay_and_direct_launch_agree(oracle64, monkeypatch, single_unit):
+    """The same plan run as a replayed graph (MINFFT_GRAPH=1), as direct
+    launches (0) and by the default rule gives identical outputs, replay
+    after replay with other device work in between, and they match the
+    oracle."""
+    torch = _torch()
+    prec = 32
+    if single_unit:
+        entries = [("dft", (1024,))] * 24
+    else:
+        entries = [("dft", (n,)) for n in (64, 512, 4096, 16384, 65536)] * 3 + \
+            [("realdft", (64, 32)), ("dct2", (32, 64)), ("invdft", (256,))]
+    xs = [make_input(k, s, prec, 900 + i) for i, (k, s) in enumerate(entries)]
+    ioffs, ooffs, ipos, opos = [], [], 0, 0
+    for (k, s), x in zip(entries, xs):
+        ioffs.append(ipos)
+        ipos += x.nbytes
+        ooffs.append(opos)
+        opos += int(np.prod(out_shape(k, s))) * (8 if is_complex_out(k) else 4)
+    hin = np.zeros(ipos, dtype=np.uint8)
+    for x, o in zip(xs, ioffs):
+        hin[o:o + x.nbytes] = np.frombuffer(np.ascontiguousarray(x).tobytes(), dtype=np.uint8)
+    din = torch.from_numpy(hin).cuda()
+    b = mf.Batch([(k, s, io, oo) for (k, s), io, oo in zip(entries, ioffs, ooffs)], mf.PREC_FP32)
+    st = torch.cuda.Stream()
+    outs = {}
+    for mode in ("1", "0", None):
+        if mode is None:
+            monkeypatch.delenv("MINFFT_GRAPH", raising=False)
+        else:
+            monkeypatch.setenv("MINFFT_GRAPH", mode)
+        dout = torch.zeros(opos, dtype=torch.uint8, device="cuda")
+        got = []
+        for rep in range(3):
+            dout.zero_()
+            b.run(din.data_ptr(), dout.data_ptr(), st.cuda_stream)
+            torch.cuda.synchronize()
+            got.append(dout.cpu().numpy().copy())
+            torch.randn(1 << 20, device="cuda").sum().item()  # other device work between replays
+        for g in got[1:]:
+            assert np.array_equal(g, got[0]), f"mode {mode}: replays differ"
+        outs[mode] = got[0]
+    b.close()
+    assert np.array_equal(outs["1"], outs["0"]) and np.array_equal(outs["1"], outs[None])
+    hout = outs["1"]
+    for i, ((k, s), x) in enumerate(zip(entries, xs)):
+        odt = np.complex64 if is_complex_out(k) else np.float32
+        n_out = int(np.prod(out_shape(k, s)))
+        y = hout[ooffs[i]:ooffs[i] + n_out * np.dtype(odt).itemsize].view(odt).reshape(out_shape(k, s))
+        ref = oracle64.run(k, x.astype(np.complex128 if is_complex_in(k) else np.float64), s)
+        assert rel_l2(y, ref) <= TOL32, (i, k, s)
