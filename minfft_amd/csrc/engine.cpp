@@ -158,8 +158,8 @@ bool is_device_pointer(const void* p) {
 
 // ------------------------------------------------------------ tile tuning
 // Process-wide schedule: defaults, then MINFFT_TUNE="key=value,..." (keys:
-// tpw, cfb, cfcore, wave, xcd, e32, stage, chunk, split, split2, fused, wpc,
-// falt, rsplit -- the fields of Tune in engine.hpp).
+// tpw, cfb, cfcore, wave, xcd, e32, e4, stage, chunk, split, split2, fused,
+// wpc, falt, rsplit -- the fields of Tune in engine.hpp).
 static Tune g_tune;
 static std::once_flag g_tune_once;
 static thread_local const Tune* t_tune = nullptr;
@@ -191,6 +191,7 @@ bool parse_tune(const char* spec, Tune& t) {
         else if (k == "split2") t.split2 = v;
         else if (k == "cfcore") t.cfcore = v;
         else if (k == "e32") t.e32 = v;
+        else if (k == "e4") t.e4 = v;
         else if (k == "fused") t.fused = v;
         else if (k == "wpc" && v > 0) t.fused_wpc = v;
         else if (k == "falt") t.fused_alt = v;
@@ -293,8 +294,14 @@ static int launch_tile(Prec prec, int kind, int logL, const void* tw, const void
     }
     split16 = split16 && prec == PREC_F32 && logL == 9 && (kind == K_DFT || kind == K_INVDFT) && fs_logN == 0;
     const bool e32 = !split16 && tune().e32 && kt.tile32[kind][logL];
-    const void* fn = e32 ? kt.tile32[kind][logL] : kt.tile[kind][logL];
-    const int E = split16 ? 16 : (e32 ? 32 : kt.tile_E[logL]);
+    // a small complex pass (a few transforms in all) is bound by latency,
+    // not bandwidth: 4 points per lane give each transform four times the
+    // lanes of the 16-point tile (one 1024-point FP64 transform: 3.5 against
+    // 4.0 us per launch back to back)
+    const bool e4 = !split16 && !e32 && tune().e4 && (kind == K_DFT || kind == K_INVDFT) && fs_logN == 0 &&
+                    kt.tile4[kind == K_INVDFT][logL] && lv.count() << logL <= (1ull << 16);
+    const void* fn = e32 ? kt.tile32[kind][logL] : (e4 ? kt.tile4[kind == K_INVDFT][logL] : kt.tile[kind][logL]);
+    const int E = split16 ? 16 : (e32 ? 32 : (e4 ? 4 : kt.tile_E[logL]));
     const int L = 1 << logL;
     const int TPT = L / E;
     TileArgs A;
@@ -367,7 +374,7 @@ static int launch_tile(Prec prec, int kind, int logL, const void* tw, const void
     // packed real pair (the DC and Nyquist columns of a real DFT's half
     // spectrum as one complex column): the c-fast direct form only; the pass
     // then covers h = 0..H-2
-    if (pack && kind == K_DFT && !split16 && !e32 && fs_logN == 0 && kt.tilepk[logL] && A.cfast &&
+    if (pack && kind == K_DFT && !split16 && !e32 && !e4 && fs_logN == 0 && kt.tilepk[logL] && A.cfast &&
         A.in_mode == IO_DIRECT && A.out_mode == IO_DIRECT && f == 3 && A.B[3] >= 2 && A.in_s[3] == 1 &&
         A.out_s[3] == 1) {
         fn = kt.tilepk[logL];

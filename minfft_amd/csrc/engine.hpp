@@ -84,6 +84,8 @@ struct Tune {
     int fused = 1;      // fused two-step kernel for batches of long 1D entries
     int fused_wpc = 2;  // work-groups per CU of the fused kernel
     int fused_alt = 0;  // use the alternative (smaller) fused work-groups
+    int e4 = 1;         // complex passes of at most 2^16 points in all (lengths
+                        // 2^5..2^10) on 4-point-per-lane tiles: 1 = on, 0 = off
     int rsplit = 0;     // residue-split single-pass kernel for contiguous axes:
                         // 10*log2 P + residues per work-group (e.g. 31 = 8 blocks,
                         // one residue), where instantiated; 0 = off

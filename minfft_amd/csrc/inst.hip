@@ -32,6 +32,9 @@ static void reg_tiles(KTable& t) {
         t.tile_E[LOGL] = E;
         if constexpr (INST_KIND == K_DFT)
             t.tilepk[LOGL] = reinterpret_cast<const void*>(&tile_kernel<IT, LOGL, E, K_DFT, true>);
+        if constexpr ((INST_KIND == K_DFT || INST_KIND == K_INVDFT) && LOGL >= kSmallLogMin &&
+                      LOGL <= kSmallLogMax)
+            t.tile4[INST_KIND == K_INVDFT][LOGL] = reinterpret_cast<const void*>(&tile_kernel<IT, LOGL, 4, INST_KIND>);
         reg_tiles<LOGL + 1>(t);
     }
 }

@@ -15,11 +15,18 @@ struct KTable {
     const void* tile32[K_COUNT][kMaxLog + 1];
     // forward complex tiles with the packed real pair (tile_kernel PACK)
     const void* tilepk[kMaxLog + 1];
+    // small-pass complex tiles, 4 points per lane ([inverse][logL], logL
+    // kSmallLogMin..kSmallLogMax; null elsewhere)
+    const void* tile4[2][kMaxLog + 1];
     int maxlog;                               // largest logL instantiated
     const void* pre[K_COUNT];
     const void* post[K_COUNT];
     const void* n1[K_COUNT];
 };
+
+// lengths of the 4-point-per-lane complex tiles
+constexpr int kSmallLogMin = 5;
+constexpr int kSmallLogMax = 10;
 
 // maximum single-tile logL compiled per precision
 constexpr int kMaxLogF32 = 14;  // 16384 x 8 B = 128 KiB + padding
