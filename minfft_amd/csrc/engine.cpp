@@ -371,13 +371,13 @@ static int launch_tile(Prec prec, int kind, int logL, const void* tw, const void
         A.in_mode = in_row ? IO_DIRECT : (in_cf ? IO_STAGE_CF : IO_STAGE_ROW);
         A.out_mode = out_row ? IO_DIRECT : (out_cf ? IO_STAGE_CF : IO_STAGE_ROW);
     }
-    // packed real pair (the DC and Nyquist columns of a real DFT's half
-    // spectrum as one complex column): the c-fast direct form only; the pass
-    // then covers h = 0..H-2
-    if (pack && kind == K_DFT && !split16 && !e32 && !e4 && fs_logN == 0 && kt.tilepk[logL] && A.cfast &&
-        A.in_mode == IO_DIRECT && A.out_mode == IO_DIRECT && f == 3 && A.B[3] >= 2 && A.in_s[3] == 1 &&
-        A.out_s[3] == 1) {
-        fn = kt.tilepk[logL];
+    // packed pair (columns h = 0 and h = H-1 of a real DFT's half spectrum
+    // as one complex column, tile.hpp): the c-fast direct form only; the
+    // pass then covers h = 0..H-2
+    if (pack && (kind == K_DFT || kind == K_INVDFT) && !split16 && !e32 && !e4 && fs_logN == 0 &&
+        kt.tilepk[kind == K_INVDFT][logL] && A.cfast && A.in_mode == IO_DIRECT && A.out_mode == IO_DIRECT &&
+        f == 3 && A.B[3] >= 2 && A.in_s[3] == 1 && A.out_s[3] == 1) {
+        fn = kt.tilepk[kind == K_INVDFT][logL];
         A.pack_in = (long long)(A.B[3] - 1);
         A.pack_out = (long long)(A.B[3] - 1);
         ntr = ntr / A.B[3] * (A.B[3] - 1);
@@ -1025,9 +1025,18 @@ int run_entries(const Workspace& ws, Prec pr, int kind, int d, const int* Ns, co
     for (int a = d - 2; a >= 0; --a) {
         long long S;
         Levels a2 = axis_levels(Hm, a, S);
+        if (a == 0 && S > H) {
+            // h as a level of its own (the packed pair below needs its index)
+            a2 = Levels();
+            a2.push((unsigned)(S / H), H, H);
+            a2.push((unsigned)H, 1, 1);
+        }
         const bool first = (a == d - 2);
-        int rc = axis_pass(ws, pr, K_INVDFT, ilog2i(Dm[a]), first ? in : (const void*)tbuf, S, tbuf, S,
-                           mk(a2, first ? ioff : toff, toff), st);
+        Levels lv = mk(a2, first ? ioff : toff, toff);
+        // the last complex pass feeds the real pass, which keeps only the
+        // real parts of h = 0 and h = H-1
+        lv.pack = (a == 0) && H >= 2;
+        int rc = axis_pass(ws, pr, K_INVDFT, ilog2i(Dm[a]), first ? in : (const void*)tbuf, S, tbuf, S, lv, st);
         if (rc) return rc;
         src = tbuf;
         soff = toff;

@@ -27,9 +27,11 @@ struct Levels {
     const long long* ioff = nullptr;
     const long long* ooff = nullptr;
     const long long* foff = nullptr;   // two-step scratch offsets for the table level
-    // forward complex pass over a real DFT's half spectrum whose last level
-    // indexes h = 0..H-1 (stride 1): columns 0 and H-1 are real, and a tile
-    // pass may transform them as one complex column
+    // complex pass over a real DFT's half spectrum whose last level indexes
+    // h = 0..H-1 (stride 1), where a tile pass may transform columns 0 and
+    // H-1 as one complex column: the forward pass right after the real pass
+    // (both columns real), or the inverse pass right before it (only their
+    // real parts are read)
     bool pack = false;
     void push(unsigned b, long long i, long long o) {
         B[n] = b;

@@ -13,8 +13,8 @@ struct KTable {
     // radix-32 variants (32 points per thread, 512-lane work-groups at most):
     // FP32, logL >= 9 only; null elsewhere
     const void* tile32[K_COUNT][kMaxLog + 1];
-    // forward complex tiles with the packed real pair (tile_kernel PACK)
-    const void* tilepk[kMaxLog + 1];
+    // complex tiles with the packed pair (tile_kernel PACK), [inverse][logL]
+    const void* tilepk[2][kMaxLog + 1];
     // small-pass complex tiles, 4 points per lane ([inverse][logL], logL
     // kSmallLogMin..kSmallLogMax; null elsewhere)
     const void* tile4[2][kMaxLog + 1];

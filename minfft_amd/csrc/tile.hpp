@@ -552,8 +552,8 @@ template <typename T> constexpr int max_threads() { return sizeof(T) == 8 ? 512 
 // radix-32 tiles (E = 32) hold twice the points per lane: at most 512 lanes
 template <typename T, int E> constexpr int tile_threads() { return E >= 32 ? 512 : max_threads<T>(); }
 
-// PACK (forward complex pass over a real DFT's half spectrum, direct c-fast
-// I/O on both sides): see `packed` below.
+// PACK (complex pass over a real DFT's half spectrum, direct c-fast I/O on
+// both sides): see `packed` below.
 template <typename T, int LOGL, int E, int KIND, bool PACK = false>
 __global__ void __launch_bounds__((tile_threads<T, E>())) tile_kernel(TileArgs A) {
     using C2 = c2<T>;
@@ -585,12 +585,16 @@ __global__ void __launch_bounds__((tile_threads<T, E>())) tile_kernel(TileArgs A
     const bool wv_out = wv && A.out_mode != IO_STAGE_CF;
 
     C2 v[E];
-    // Packed real pair: after a real DFT along the last axis, the columns
-    // h = 0 and h = H-1 of the half spectrum are real (the DC and Nyquist
-    // terms of every row), so the transform at h = 0 takes column H-1 as its
-    // imaginary part, z = a + i b, and the pass skips h = H-1.  With
-    // Z = DFT(z): A[k] = (Z[k] + conj Z[-k]) / 2, B[k] = (Z[k] - conj Z[-k]) / 2i.
-    static_assert(!PACK || KIND == K_DFT, "packed pairs: forward complex passes");
+    // Packed pair: the transform at h = 0 also carries column h = H-1 of a
+    // real DFT's half spectrum, and the pass skips h = H-1.
+    // Forward (after the real DFT along the last axis): both columns are real
+    // (the DC and Nyquist terms of every row), so z = a + i b and, with
+    // Z = DFT(z), A[k] = (Z[k] + conj Z[-k]) / 2, B[k] = (Z[k] - conj Z[-k]) / 2i.
+    // Inverse (the pass whose output the inverse real DFT along the last axis
+    // reads, which keeps only the real parts of h = 0 and h = H-1):
+    // Re IDFT(c) = IDFT(c_h) with the Hermitian part c_h[k] = (c[k] + conj c[-k]) / 2,
+    // and IDFT(a_h + i b_h) = Re IDFT(a) + i Re IDFT(b).
+    static_assert(!PACK || KIND == K_DFT || KIND == K_INVDFT, "packed pairs: complex passes");
     const bool packed = PACK && me.valid && me.i3 == 0u;
 
     // ---------------- input side ----------------
@@ -601,7 +605,7 @@ __global__ void __launch_bounds__((tile_threads<T, E>())) tile_kernel(TileArgs A
         // complex input read straight into the owned positions
         const C2* __restrict__ in = reinterpret_cast<const C2*>(A.in) + me.ib;
         if (me.valid) {
-            if (packed) {
+            if (KIND == K_DFT && packed) {
                 // real parts of the two columns (reals at twice the item stride)
                 const T* __restrict__ ra = reinterpret_cast<const T*>(in + (long long)tl * A.in_e);
                 const T* __restrict__ rb = reinterpret_cast<const T*>(in + A.pack_in + (long long)tl * A.in_e);
@@ -621,6 +625,39 @@ __global__ void __launch_bounds__((tile_threads<T, E>())) tile_kernel(TileArgs A
         } else {
 #pragma unroll
             for (int m = 0; m < E; ++m) v[m] = mk<T>(0, 0);
+        }
+        if constexpr (PACK && KIND == K_INVDFT) {
+            // Hermitian parts of the two columns through the slot (every
+            // lane reaches the barriers; the slot is free before the core)
+            const T h = (T)0.5;
+            C2 u[E];
+            if (packed) {
+                const C2* __restrict__ b = in + A.pack_in + (long long)tl * A.in_e;
+                const long long step = A.in_e * TPT;
+#pragma unroll
+                for (int m = 0; m < E; ++m) u[m] = b[m * step];
+                Core<T, LOGL, E, INV>::write_owned(v, slot, tl);
+            }
+            __syncthreads();
+            if (packed) {
+#pragma unroll
+                for (int m = 0; m < E; ++m) {
+                    const C2 w = slot[lpad((L - tl - m * TPT) & (L - 1))];
+                    v[m] = mk<T>((v[m].x + w.x) * h, (v[m].y - w.y) * h);
+                }
+            }
+            __syncthreads();
+            if (packed) Core<T, LOGL, E, INV>::write_owned(u, slot, tl);
+            __syncthreads();
+            if (packed) {
+#pragma unroll
+                for (int m = 0; m < E; ++m) {
+                    const C2 w = slot[lpad((L - tl - m * TPT) & (L - 1))];
+                    const T br = (u[m].x + w.x) * h, bi = (u[m].y - w.y) * h;
+                    v[m] = mk<T>(v[m].x - bi, v[m].y + br);
+                }
+            }
+            __syncthreads();
         }
     } else {
         const bool rowm = (A.in_mode == IO_STAGE_ROW);
@@ -672,7 +709,22 @@ __global__ void __launch_bounds__((tile_threads<T, E>())) tile_kernel(TileArgs A
     if constexpr (KIND == K_DFT || KIND == K_INVDFT) {
         if (A.out_mode == IO_DIRECT) {
             C2* __restrict__ out = reinterpret_cast<C2*>(A.out) + me.ob;
-            if constexpr (PACK) {
+            if constexpr (PACK && KIND == K_INVDFT) {
+                if (packed) {
+                    // real parts only: Re IDFT of column 0 and of column H-1
+                    const long long es = A.out_e;
+                    C2* __restrict__ oa = out + (long long)tl * es;
+                    C2* __restrict__ ob = out + A.pack_out + (long long)tl * es;
+                    const long long step = es * TPT;
+#pragma unroll
+                    for (int m = 0; m < E; ++m) {
+                        oa[m * step] = mk<T>(v[m].x, (T)0);
+                        ob[m * step] = mk<T>(v[m].y, (T)0);
+                    }
+                    return;
+                }
+            }
+            if constexpr (PACK && KIND == K_DFT) {
                 {  // every lane reaches the barrier
                     // Z through the slot (free: the core ends with a barrier
                     // after its last slot read) to pair Z[k] with Z[-k]

@@ -129,6 +129,30 @@ def test_realdft_packed_dc_nyquist(libs, oracle64, prec, shape):
         assert rel_l2(col, flip) <= 10 * bar
 
 
+@pytest.mark.parametrize("prec", [64, 32])
+@pytest.mark.parametrize("shape", [(2, 2), (1, 16), (4096, 4), (16384, 2), (8, 4, 2), (2, 1, 8), (4, 8, 16),
+                                   (2048, 2048)], ids=lambda s: "x".join(map(str, s)))
+def test_invrealdft_packed_dc_nyquist(libs, oracle64, prec, shape):
+    """The last complex pass of a multi-dimensional inverse real DFT
+    transforms the Hermitian parts of columns h = 0 and h = H-1 as one
+    complex column (the real pass after it reads only their real parts).
+    Inputs are arbitrary complex arrays, not spectra of real arrays, so the
+    imaginary parts the reference ignores are not zero; against the oracle,
+    with the input preserved."""
+    print(f"[case] invrealdft packed pair prec={prec} shape={shape}", flush=True)
+    rng = np.random.default_rng(710 + len(shape) + shape[-1])
+    ish = in_shape("invrealdft", shape)
+    cdt = np.complex128 if prec == 64 else np.complex64
+    x = (rng.uniform(-1, 1, ish) + 1j * rng.uniform(-1, 1, ish)).astype(cdt)
+    x0 = x.copy()
+    y = gpu_run(libs[prec], "invrealdft", x, shape)
+    assert np.array_equal(x, x0)
+    yo = oracle64.run("invrealdft", x.astype(np.complex128), shape)
+    bar = tol64(shape) if prec == 64 else TOL32
+    e = rel_l2(y, yo)
+    assert e <= bar, f"{shape} prec={prec}: rel-L2 {e:.3e} > {bar:.1e}"
+
+
 @pytest.mark.parametrize("prec,N", [(64, 16384), (64, 32768), (64, 131072), (32, 32768), (32, 65536),
                                     (32, 262144)])
 def test_long_1d_two_step(libs, oracle64, prec, N):
