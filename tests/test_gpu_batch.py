@@ -26,6 +26,8 @@ def run_batch(entries, prec, xs, oracle64, check_idx=None, seed=0, mode=None):
     ioffs, ooffs = [], []
     ipos = opos = 0
     for (kind, shape), x in zip(entries, xs):
+        ipos = (ipos + 15) & ~15  # entries start on 16-byte boundaries (any item size)
+        opos = (opos + 15) & ~15
         ioffs.append(ipos)
         ipos += x.nbytes + 64 * rs  # gap between entries
         n_out = int(np.prod(out_shape(kind, shape))) * (2 if is_complex_out(kind) else 1)
@@ -355,3 +357,30 @@ def test_graph_replay_and_direct_launch_agree(oracle64, monkeypatch, single_unit
         y = hout[ooffs[i]:ooffs[i] + n_out * np.dtype(odt).itemsize].view(odt).reshape(out_shape(k, s))
         ref = oracle64.run(k, x.astype(np.complex128 if is_complex_in(k) else np.float64), s)
         assert rel_l2(y, ref) <= TOL32, (i, k, s)
+
+
+ALL_KINDS = ["dft", "invdft", "realdft", "invrealdft", "dct2", "dst2", "dct3", "dst3", "dct4", "dst4"]
+
+
+@pytest.mark.parametrize("prec", [32, 64])
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_random_mixed_batches(oracle64, prec, seed):
+    """Seeded random batches: every kind, 1 to 3 dimensions, 2^0..2^11
+    points per axis (at most 2^16 per entry), shapes repeated 1 to 12 times
+    (groups of one or several entries: persistent launch, small-pass tiles,
+    packed DC/Nyquist columns, graph or direct launch), every entry against
+    the oracle and the gaps between outputs untouched."""
+    rng = np.random.default_rng(1000 + seed)
+    entries, xs = [], []
+    while len(entries) < 60:
+        kind = ALL_KINDS[int(rng.integers(len(ALL_KINDS)))]
+        d = int(rng.integers(1, 4))
+        while True:
+            shape = tuple(1 << int(rng.integers(0, 12)) for _ in range(d))
+            if int(np.prod(shape)) <= 1 << 16:
+                break
+        for _ in range(int(rng.choice([1, 1, 2, 5, 12]))):
+            entries.append((kind, shape))
+            xs.append(make_input(kind, shape, prec, int(rng.integers(1 << 30)), oracle64))
+    print(f"[case] random batch seed={seed} prec={prec}: {len(entries)} entries", flush=True)
+    run_batch(entries, prec, xs, oracle64)
