@@ -193,8 +193,10 @@ TuneResult autotune_group(const Workspace& ws, const TuneRequest& r, hipStream_t
         hio[i] = (long long)i * ii;
         hoo[i] = (long long)i * oi;
     }
-    if (hipMalloc(&din, (size_t)cs * ii * isz) != hipSuccess || hipMalloc(&dout, (size_t)cs * oi * osz) != hipSuccess ||
-        hipMalloc(&dio, cs * sizeof(long long)) != hipSuccess || hipMalloc(&doo, cs * sizeof(long long)) != hipSuccess ||
+    if (dev_alloc((void**)&din, (size_t)cs * ii * isz) != hipSuccess ||
+        dev_alloc((void**)&dout, (size_t)cs * oi * osz) != hipSuccess ||
+        dev_alloc((void**)&dio, cs * sizeof(long long)) != hipSuccess ||
+        dev_alloc((void**)&doo, cs * sizeof(long long)) != hipSuccess ||
         hipMemcpy(dio, hio.data(), cs * sizeof(long long), hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(doo, hoo.data(), cs * sizeof(long long), hipMemcpyHostToDevice) != hipSuccess ||
         hipMemsetAsync(din, 0, (size_t)cs * ii * isz, st) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||

@@ -22,6 +22,7 @@
 #include <cstring>
 #include <map>
 #include <tuple>
+#include <utility>
 #include <vector>
 
 #include "autotune.hpp"
@@ -242,15 +243,20 @@ minfft_batch_plan* minfft_mkaux_batch(int count, const minfft_batch_entry* e, in
             g.tbytes = (size_t)cnt * (size_t)(rows * H) * cs;
             g.toff.resize(cnt);
             for (unsigned i = 0; i < cnt; ++i) g.toff[i] = (long long)i * rows * H;
-            if (hipMalloc(&g.tbuf, g.tbytes) != hipSuccess) return fail("hipMalloc batch temp failed");
+            if (dev_alloc((void**)&g.tbuf, g.tbytes) != hipSuccess) return fail("hipMalloc batch temp failed");
         }
-        auto up = [&](std::vector<long long>& h, long long*& dptr) -> bool {
-            if (h.empty()) return true;
-            if (hipMalloc(&dptr, h.size() * sizeof(long long)) != hipSuccess) return false;
-            return hipMemcpy(dptr, h.data(), h.size() * sizeof(long long), hipMemcpyHostToDevice) == hipSuccess;
+        // the group's offset tables on the device (nullptr: done, else the reason)
+        auto up = [&](std::vector<long long>& h, long long*& dptr) -> const char* {
+            if (h.empty()) return nullptr;
+            if (dev_alloc((void**)&dptr, h.size() * sizeof(long long)) != hipSuccess)
+                return "hipMalloc batch tables failed";
+            if (hipMemcpy(dptr, h.data(), h.size() * sizeof(long long), hipMemcpyHostToDevice) != hipSuccess)
+                return "uploading batch tables failed";
+            return nullptr;
         };
-        if (!up(g.ioff, g.d_ioff) || !up(g.ooff, g.d_ooff) || !up(g.foff, g.d_foff) || !up(g.toff, g.d_toff))
-            return fail("uploading batch tables failed");
+        for (auto t : {std::make_pair(&g.ioff, &g.d_ioff), std::make_pair(&g.ooff, &g.d_ooff),
+                       std::make_pair(&g.foff, &g.d_foff), std::make_pair(&g.toff, &g.d_toff)})
+            if (const char* why = up(*t.first, *t.second)) return fail(why);
     }
     // largest groups first
     std::stable_sort(p->groups.begin(), p->groups.end(), [](const BatchGroup& a, const BatchGroup& b) {
@@ -262,8 +268,8 @@ minfft_batch_plan* minfft_mkaux_batch(int count, const minfft_batch_entry* e, in
     if (alloc_twiddles(p->ws, p->prec, twlog, p->stream)) return fail(nullptr);
     p->ws.split16 = (prec == MINFFT_SPLIT_FP16) ? 1 : 0;
     if (fs_max) {
-        if (hipMalloc(&p->ws.fbuf, fs_max) != hipSuccess) return fail("hipMalloc batch scratch failed");
-        if (hipMalloc(&p->ws.gbuf, fs_max) != hipSuccess) return fail("hipMalloc batch scratch failed");
+        if (dev_alloc((void**)&p->ws.fbuf, fs_max) != hipSuccess) return fail("hipMalloc batch scratch failed");
+        if (dev_alloc((void**)&p->ws.gbuf, fs_max) != hipSuccess) return fail("hipMalloc batch scratch failed");
         p->ws.fs_bytes = fs_max;
     }
     if (hipStreamSynchronize(p->stream) != hipSuccess) return fail("twiddle generation failed");
@@ -310,7 +316,8 @@ minfft_batch_plan* minfft_mkaux_batch(int count, const minfft_batch_entry* e, in
         p->multi.push_back(ml);
     }
     if (!p->multi.empty()) {
-        if (hipMalloc(&p->d_counter, 2 * sizeof(unsigned)) != hipSuccess) return fail("hipMalloc batch counter failed");
+        if (dev_alloc((void**)&p->d_counter, 2 * sizeof(unsigned)) != hipSuccess)
+            return fail("hipMalloc batch counter failed");
         for (size_t k = 0; k < p->multi.size(); ++k) p->multi[k].args.counter = p->d_counter + k;
     }
     for (auto& g : p->groups) {
@@ -535,7 +542,7 @@ int minfft_batch(minfft_batch_plan* p, const void* in, size_t in_bytes, void* ou
         if (buf) (void)hipFree(buf);
         buf = nullptr;
         have = 0;
-        if (hipMalloc(&buf, need) != hipSuccess) return false;
+        if (dev_alloc((void**)&buf, need) != hipSuccess) return false;
         have = need;
         return true;
     };

@@ -109,6 +109,26 @@ static std::mutex g_dev_mu;
 
 Device& device() { return g_dev; }
 
+hipError_t dev_alloc(void** p, size_t bytes) {
+    const char* e = std::getenv("MINFFT_FAIL_ALLOC");
+    if (e && *e) {
+        static std::mutex mu;
+        static std::string seen;
+        static long long count = 0;
+        std::lock_guard<std::mutex> lk(mu);
+        if (seen != e) {
+            seen = e;
+            count = 0;
+        }
+        const long long k = std::atoll(e);
+        if (k > 0 && ++count >= k) {
+            *p = nullptr;
+            return hipErrorOutOfMemory;
+        }
+    }
+    return hipMalloc(p, bytes);
+}
+
 bool ensure_device() {
     std::lock_guard<std::mutex> lk(g_dev_mu);
     int cur = -1;
@@ -782,9 +802,9 @@ int axis_pass(const Workspace& ws, Prec prec, int kind, int logN, const void* in
 // ------------------------------------------------------------------ plans
 int alloc_twiddles(Workspace& ws, Prec prec, int log_m, hipStream_t st) {
     const long long total = 2LL << log_m;
-    HIPCHK(hipMalloc(&ws.tw64, (size_t)total * sizeof(double2)), "hipMalloc twiddles");
+    HIPCHK(dev_alloc((void**)&ws.tw64, (size_t)total * sizeof(double2)), "hipMalloc twiddles");
     if (prec == PREC_F32) {
-        HIPCHK(hipMalloc(&ws.tw, (size_t)total * sizeof(float2)), "hipMalloc twiddles");
+        HIPCHK(dev_alloc((void**)&ws.tw, (size_t)total * sizeof(float2)), "hipMalloc twiddles");
     } else {
         ws.tw = ws.tw64;
     }
@@ -859,6 +879,7 @@ Plan* make_plan(Family fam, Prec prec, int d, const int* Ns) {
     p->dev = device().id;
     auto fail = [&](const char* what) -> Plan* {
         if (what && !*last_error()) set_error(what);
+        (void)hipGetLastError();  // the failed call's error stays with this maker
         free_plan(p);
         return nullptr;
     };
@@ -895,13 +916,13 @@ Plan* make_plan(Family fam, Prec prec, int d, const int* Ns) {
     (void)cs;
     if (alloc_twiddles(p->ws, prec, twlog, p->stream)) return fail(nullptr);
     if (fs) {
-        if (hipMalloc(&p->ws.fbuf, fs) != hipSuccess) return fail("hipMalloc scratch failed");
-        if (hipMalloc(&p->ws.gbuf, fs) != hipSuccess) return fail("hipMalloc scratch failed");
+        if (dev_alloc((void**)&p->ws.fbuf, fs) != hipSuccess) return fail("hipMalloc scratch failed");
+        if (dev_alloc((void**)&p->ws.gbuf, fs) != hipSuccess) return fail("hipMalloc scratch failed");
         p->ws.fs_bytes = fs;
     }
     if (fam == FAM_REALDFT && d > 1) {
         p->tbuf_bytes = (size_t)(rows * H) * 2 * rs;
-        if (hipMalloc(&p->tbuf, p->tbuf_bytes) != hipSuccess) return fail("hipMalloc scratch failed");
+        if (dev_alloc((void**)&p->tbuf, p->tbuf_bytes) != hipSuccess) return fail("hipMalloc scratch failed");
     }
     if (hipStreamSynchronize(p->stream) != hipSuccess) return fail("twiddle generation failed");
     // launch schedule: timed on the device for large plans, with the
@@ -1086,7 +1107,7 @@ int execute(Plan& p, int kind, const void* in, void* out) {
         if (p.stage) (void)hipFree(p.stage);
         p.stage = nullptr;
         p.stage_bytes = 0;
-        HIPCHK(hipMalloc(&p.stage, need), "hipMalloc staging buffer");
+        HIPCHK(dev_alloc((void**)&p.stage, need), "hipMalloc staging buffer");
         p.stage_bytes = need;
     }
     // Host arrays up to kPinnedStageMax go through a pinned bounce buffer

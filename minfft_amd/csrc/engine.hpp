@@ -138,6 +138,12 @@ void clear_error();
 Device& device();
 bool ensure_device();
 
+// Device allocation of the library (plans, scratch, staging, tuning
+// buffers).  Test hook: with MINFFT_FAIL_ALLOC=k set, the k-th allocation
+// after the variable took its current value, and every later one, fail as
+// out of memory (the error paths of the makers and transforms).
+hipError_t dev_alloc(void** p, size_t bytes);
+
 size_t real_size(Prec p);
 Plan* make_plan(Family fam, Prec prec, int d, const int* Ns);
 void free_plan(Plan* p);

@@ -289,12 +289,26 @@ def test_split_fp16_keeps_512_on_matrix_cores():
     assert l256 and "persistent" in l256[0], lines
 
 
+def _warm_device():
+    """Load the libraries' kernels and the runtime's own buffers once, so
+    that device-memory readings before and after a test compare plans only."""
+    torch = _torch()
+    x = torch.zeros(1 << 16, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    for prec in (mf.PREC_FP64, mf.PREC_FP32):
+        b = mf.Batch([("invrealdft", (64, 64), 0, 0), ("dft", (1 << 15,), 0, 0)], prec)
+        b.run(x.data_ptr(), y.data_ptr(), None)
+        torch.cuda.synchronize()
+        b.close()
+    torch.cuda.synchronize()
+
+
 def test_batch_allocation_failure_returns_null_without_leak():
     """A plan whose scratch exceeds the device memory (3000 inverse real DFTs
     of 4096^2 FP64 need ~400 GB of complex staging) is refused: the maker
     returns NULL with a reason, and frees what it had allocated."""
     import torch
-    torch.cuda.synchronize()
+    _warm_device()
     free0, total = torch.cuda.mem_get_info()
     need = 3000 * 4096 * 2049 * 16
     assert need > total
@@ -384,3 +398,65 @@ def test_random_mixed_batches(oracle64, prec, seed):
             xs.append(make_input(kind, shape, prec, int(rng.integers(1 << 30)), oracle64))
     print(f"[case] random batch seed={seed} prec={prec}: {len(entries)} entries", flush=True)
     run_batch(entries, prec, xs, oracle64)
+
+
+def test_allocation_failure_hook(oracle64, monkeypatch):
+    """With MINFFT_FAIL_ALLOC=k the k-th device allocation and every later
+    one fail as out of memory.  For each k, the plain maker (twiddles,
+    two-step scratch, real-DFT temporary, tuning buffers) and the batch
+    maker (offset tables, scratch, temporaries, counter, tuning buffers)
+    either return NULL with the reason or succeed; a failing transform
+    reports its error; nothing is left on the device, and plans made
+    afterwards compute correctly."""
+    torch = _torch()
+    lib = mf.Minfft(64)
+    _warm_device()
+    lib.free_aux(lib.mkaux_raw("minfft_mkaux_realdft_2d", 64, 64))
+    free0 = torch.cuda.mem_get_info()[0]
+    n16 = 1 << 16
+    bent = [("dft", (n16,), i * n16 * 8, i * n16 * 8) for i in range(260)]
+    bent += [("invrealdft", (64, 128), 0, 0), ("dft", (256,), 0, 0), ("dft", (4096,), 0, 0)]
+
+    def make_plain():
+        return lib.mkaux_raw("minfft_mkaux_realdft_2d", 1 << 15, 64)
+
+    def make_batch():
+        try:
+            return mf.Batch(bent, mf.PREC_FP32)
+        except mf.MinfftError:
+            return None
+
+    for name, make, free in (("plain", make_plain, lib.free_aux), ("batch", make_batch, lambda b: b.close())):
+        outcomes = []
+        for k in range(1, 41):
+            monkeypatch.setenv("MINFFT_FAIL_ALLOC", str(k))
+            h = make()
+            if h:
+                free(h)
+            else:
+                err = mf.last_error()
+                assert "hipMalloc" in err or "memory" in err.lower(), (name, k, err)
+            outcomes.append(bool(h))
+        monkeypatch.delenv("MINFFT_FAIL_ALLOC")
+        print(f"[case] {name}: succeeds from k = {outcomes.index(True) + 1 if True in outcomes else None}",
+              flush=True)
+        assert not outcomes[0] and outcomes[-1], (name, outcomes)
+        first = outcomes.index(True)
+        assert all(outcomes[first:]), (name, outcomes)
+
+    # a transform whose host-array staging cannot be allocated reports it
+    shape = (64, 64)
+    x = make_input("dft", shape, 64, 17)
+    aux = mf.Aux(lib, "dft", shape)
+    y = np.empty_like(x)
+    monkeypatch.setenv("MINFFT_FAIL_ALLOC", "1")
+    with pytest.raises(mf.MinfftError):
+        aux("dft", x, y)
+    monkeypatch.delenv("MINFFT_FAIL_ALLOC")
+    aux("dft", x, y)
+    assert rel_l2(y, oracle64.run("dft", x, shape)) <= tol64(shape)
+    aux.close()
+
+    torch.cuda.synchronize()
+    free1 = torch.cuda.mem_get_info()[0]
+    assert free1 >= free0 - (64 << 20), (free0, free1)
