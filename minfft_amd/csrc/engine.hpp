@@ -107,6 +107,8 @@ struct Plan {
     size_t stage_bytes = 0;
     void* hstage = nullptr;   // pinned host bounce buffer for host pointers (lazy,
     size_t hstage_bytes = 0;  // up to kPinnedStageMax bytes)
+    unsigned* hflag = nullptr;  // completion word in coherent pinned memory (lazy)
+    unsigned hseq = 0;          // last value queued for it
     hipStream_t stream = nullptr;
     bool own_stream = true;
     Tune sched;               // launch schedule (autotuned for large plans)

@@ -61,6 +61,21 @@ __global__ void twiddle_kernel(double2* __restrict__ tw64, float2* __restrict__ 
     if (tw32) tw32[i] = make_float2((float)w.x, (float)w.y);
 }
 
+// Completion word for the host: written once every earlier launch of the
+// stream has finished (stream order), after a system-scope fence, into
+// coherent pinned host memory, so a host thread can poll it instead of
+// synchronising the stream.
+__global__ void host_flag_kernel(unsigned* flag, unsigned v) {
+    __threadfence_system();
+    __atomic_store_n(flag, v, __ATOMIC_RELEASE);
+}
+
+hipError_t launch_host_flag(unsigned* flag, unsigned v, hipStream_t st) {
+    (void)hipGetLastError();
+    hipLaunchKernelGGL(host_flag_kernel, dim3(1), dim3(1), 0, st, flag, v);
+    return hipGetLastError();
+}
+
 hipError_t launch_twiddles(double2* tw64, float2* tw32, long long total, hipStream_t st) {
     const int bs = 256;
     long long grid = (total + bs - 1) / bs;
