@@ -1121,6 +1121,9 @@ static int wait_kernels(Plan& p) {
                 for (;;) {
                     if (__atomic_load_n(p.hflag, __ATOMIC_ACQUIRE) == v) return 0;
                     if (std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(2)) break;
+#if defined(__x86_64__)
+                    __builtin_ia32_pause();
+#endif
                 }
             } else {
                 (void)hipGetLastError();
