@@ -368,3 +368,33 @@ def test_threads_with_distinct_aux(libs, oracle64):
         assert not t.is_alive(), "a worker thread did not finish"
     for i, e in enumerate(errors):
         assert e is None, f"case {cases[i]}: {e!r}"
+
+
+@pytest.mark.parametrize("shape", [(256,), (16, 16), (4, 8, 16), (32768,)], ids=lambda s: "x".join(map(str, s)))
+def test_back_to_back_synchronous_calls(libs, shape):
+    """Every synchronous call returns with its own result in place: 60
+    consecutive calls on fresh host arrays (short ones: kernels on the
+    pinned buffer, then the polled completion word; 32768 points: DMA
+    staging) and on device arrays, each checked against numpy at once."""
+    import torch
+    print(f"[case] back-to-back calls {shape}", flush=True)
+    lib = libs[64]
+    aux = mf.Aux(lib, "dft", shape)
+    rng = np.random.default_rng(31 + len(shape))
+    n = int(np.prod(shape))
+    for i in range(60):
+        x = (rng.uniform(-1, 1, shape) + 1j * rng.uniform(-1, 1, shape)).astype(np.complex128)
+        ref = np.fft.fftn(x)
+        if i % 3 == 2:
+            xd = torch.from_numpy(x).cuda()
+            yd = torch.empty_like(xd)
+            aux("dft", xd, yd)
+            y = yd.cpu().numpy()
+        else:
+            y = np.empty_like(x)
+            aux("dft", x, y)
+        assert rel_l2(y, ref) <= tol64(shape), (i, rel_l2(y, ref))
+        back = np.empty_like(x)
+        aux("invdft", y, back)
+        assert rel_l2(back / n, x) <= tol64(shape), i
+    aux.close()
