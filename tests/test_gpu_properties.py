@@ -249,3 +249,45 @@ def test_round_trip_dct_images_fp32():
     inv.close()
     err = float(torch.linalg.vector_norm(z / (2.0 * n) ** 2 - x) / torch.linalg.vector_norm(x))
     assert err <= 2 * TOL32, err
+
+
+@pytest.mark.parametrize("prec,logn", [(64, 26), (32, 28)])
+def test_largest_1d_lengths(libs, prec, logn):
+    """The longest 1D transforms a plan accepts (the two-step limit: FP64
+    2^26, FP32 2^28 points): two output bins against direct FP64 sums on
+    the host (exact integer phases), the round trip invdft(dft(x)) = N x and
+    Parseval on the device; twice the length is refused by the maker."""
+    torch = _torch()
+    n = 1 << logn
+    rdt = torch.float64 if prec == 64 else torch.float32
+    g = torch.Generator(device="cuda")
+    g.manual_seed(logn)
+    x = torch.complex(torch.rand(n, generator=g, device="cuda", dtype=rdt) * 2 - 1,
+                      torch.rand(n, generator=g, device="cuda", dtype=rdt) * 2 - 1)
+    y = torch.empty_like(x)
+    z = torch.empty_like(x)
+    print(f"[case] 1D length 2^{logn} prec={prec}", flush=True)
+    aux = mf.Aux(libs[prec], "dft", (n,))
+    aux.enqueue("dft", x, y)
+    aux.enqueue("invdft", y, z)
+    torch.cuda.synchronize()
+    aux.close()
+    err = float(torch.linalg.vector_norm(z / n - x) / torch.linalg.vector_norm(x))
+    assert err <= (tol64((n,)) if prec == 64 else 2 * TOL32), err
+    ex = float(torch.sum(torch.abs(x.to(torch.complex128)) ** 2))
+    ey = float(torch.sum(torch.abs(y.to(torch.complex128)) ** 2))
+    assert abs(ey - n * ex) <= (1e-12 if prec == 64 else 1e-5) * n * ex
+    rms = math.sqrt(ey / n)
+    xh = x.cpu().numpy().astype(np.complex128)
+    yh = y.cpu().numpy()
+    del x, y, z
+    chunk = 1 << 24
+    for k in (1, n // 2 + 3):
+        acc = 0j
+        for s in range(0, n, chunk):
+            m = (np.arange(s, s + chunk, dtype=np.int64) * k) % n  # exact phase index
+            acc += np.sum(xh[s:s + chunk] * np.exp(-2j * np.pi * (m / n)))
+        e = abs(complex(yh[k]) - acc) / rms
+        assert e <= (1e-12 if prec == 64 else 1e-5), (k, e)
+    assert not libs[prec].mkaux_raw("minfft_mkaux_dft_1d", 2 * n)
+    assert "limit" in mf.last_error(), mf.last_error()
