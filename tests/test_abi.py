@@ -74,6 +74,32 @@ def _build_c_app(tmp_path, single):
     return exe
 
 
+# the other client programs: C99 complex numbers (<complex.h> before
+# minfft.h) and C++ std::complex arrays
+CLIENTS = {"c99": ("gcc", ["-std=c99"], "app_c99.c", ["-lm"]),
+           "cpp": ("g++", ["-std=c++11"], "app_cpp.cpp", [])}
+
+
+def _build_client(tmp_path, which, single):
+    cc, std, src, libs = CLIENTS[which]
+    lib_dir = os.path.dirname(mf.lib_path("libminfft.so"))
+    exe = os.path.join(str(tmp_path), f"{which}{int(single)}")
+    cmd = [cc] + std + ["-pedantic", "-Wall", "-Wextra", "-Werror", "-O2", f"-DMINFFT_SINGLE={int(single)}",
+                        "-I", INC, os.path.join(ROOT, "tests", "c_app", src), "-L", lib_dir,
+                        "-lminfft_single" if single else "-lminfft", f"-Wl,-rpath,{lib_dir}"] + libs + ["-o", exe]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    return exe
+
+
 @pytest.mark.parametrize("single", [False, True])
 def test_c_application_builds_against_the_dropin(tmp_path, single):
     assert os.path.exists(_build_c_app(tmp_path, single))
+
+
+@pytest.mark.parametrize("single", [False, True])
+@pytest.mark.parametrize("which", sorted(CLIENTS))
+def test_c99_and_cpp_clients_build_against_the_dropin(tmp_path, which, single):
+    """minfft.h with C99 complex numbers and from C++ (extern "C"), both
+    precisions, warnings as errors."""
+    assert os.path.exists(_build_client(tmp_path, which, single))

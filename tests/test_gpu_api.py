@@ -277,6 +277,20 @@ def test_c_application_runs_on_the_dropin(tmp_path, single):
     assert r.returncode == 0 and r.stdout.strip().endswith("ok"), (r.stdout, r.stderr)
 
 
+@pytest.mark.parametrize("single", [False, True])
+@pytest.mark.parametrize("which", ["c99", "cpp"])
+def test_c99_and_cpp_clients_run_on_the_dropin(tmp_path, which, single):
+    """A C99 program with C99 complex numbers (2D DFT against a direct sum,
+    real-DFT round trip) and a C++ program with std::complex arrays (3D DFT
+    against a direct sum, DCT-2/DCT-3 round trip) run on the GPU."""
+    import subprocess
+    from tests.test_abi import _build_client
+    exe = _build_client(tmp_path, which, single)
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    print(r.stdout)
+    assert r.returncode == 0 and r.stdout.strip().endswith("ok"), (r.stdout, r.stderr)
+
+
 def test_mixed_host_and_device_pointers(libs, oracle64):
     """One side a host array, the other a device array (staged through the
     aux's pinned bounce buffer on the host side); in place on a host array;
