@@ -460,3 +460,51 @@ def test_allocation_failure_hook(oracle64, monkeypatch):
     torch.cuda.synchronize()
     free1 = torch.cuda.mem_get_info()[0]
     assert free1 >= free0 - (64 << 20), (free0, free1)
+
+
+def test_distinct_plans_run_concurrently(oracle64):
+    """Two batch plans (each with its own scratch, counters and graph) run
+    at the same time on two streams, five times each without waiting in
+    between, give the same outputs as one run of each alone, and those match
+    the oracle."""
+    torch = _torch()
+    rng = np.random.default_rng(44)
+    plans = []
+    for p in range(2):
+        entries = [("dft", (1 << 15,))] * 8 + [("dft", (1 << 9,))] * 64 + [("dct2", (64, 64))] * 4 + \
+            [("invdft", (1 << (12 + p),))] * 16
+        xs = [make_input(k, s, 32, int(rng.integers(1 << 30))) for k, s in entries]
+        ioffs, ooffs, ipos, opos = [], [], 0, 0
+        for (k, s), x in zip(entries, xs):
+            ioffs.append(ipos)
+            ipos += x.nbytes
+            ooffs.append(opos)
+            opos += int(np.prod(out_shape(k, s))) * (8 if is_complex_out(k) else 4)
+        hin = np.zeros(ipos, dtype=np.uint8)
+        for x, o in zip(xs, ioffs):
+            hin[o:o + x.nbytes] = np.frombuffer(np.ascontiguousarray(x).tobytes(), dtype=np.uint8)
+        b = mf.Batch([(k, s, io, oo) for (k, s), io, oo in zip(entries, ioffs, ooffs)], mf.PREC_FP32)
+        plans.append(dict(entries=entries, xs=xs, ooffs=ooffs, b=b, din=torch.from_numpy(hin).cuda(),
+                          dout=torch.zeros(opos, dtype=torch.uint8, device="cuda"), st=torch.cuda.Stream()))
+    alone = []
+    for p in plans:
+        p["b"].run(p["din"].data_ptr(), p["dout"].data_ptr(), p["st"].cuda_stream)
+        torch.cuda.synchronize()
+        alone.append(p["dout"].cpu().numpy().copy())
+        p["dout"].zero_()
+    torch.cuda.synchronize()
+    for _ in range(5):
+        for p in plans:
+            p["b"].run(p["din"].data_ptr(), p["dout"].data_ptr(), p["st"].cuda_stream)
+    torch.cuda.synchronize()
+    for p, a in zip(plans, alone):
+        assert np.array_equal(p["dout"].cpu().numpy(), a)
+        for i, ((k, s), x) in enumerate(zip(p["entries"], p["xs"])):
+            if i % 7:
+                continue
+            odt = np.complex64 if is_complex_out(k) else np.float32
+            n_out = int(np.prod(out_shape(k, s)))
+            y = a[p["ooffs"][i]:p["ooffs"][i] + n_out * np.dtype(odt).itemsize].view(odt).reshape(out_shape(k, s))
+            ref = oracle64.run(k, x.astype(np.complex128 if is_complex_in(k) else np.float64), s)
+            assert rel_l2(y, ref) <= TOL32, (i, k, s)
+        p["b"].close()
