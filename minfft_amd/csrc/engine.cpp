@@ -1115,7 +1115,8 @@ static int wait_kernels(Plan& p) {
             }
         }
         if (p.hflag) {
-            const unsigned v = ++p.hseq;
+            if (++p.hseq == 0u) ++p.hseq;  // 0 is the word's initial value
+            const unsigned v = p.hseq;
             if (launch_host_flag(p.hflag, v, p.stream) == hipSuccess) {
                 const auto t0 = std::chrono::steady_clock::now();
                 for (;;) {
