@@ -103,3 +103,18 @@ def test_c99_and_cpp_clients_build_against_the_dropin(tmp_path, which, single):
     """minfft.h with C99 complex numbers and from C++ (extern "C"), both
     precisions, warnings as errors."""
     assert os.path.exists(_build_client(tmp_path, which, single))
+
+
+def test_makers_fail_loudly_without_a_device():
+    """With no GPU visible the makers return NULL and give the reason: there
+    is no CPU fallback behind the API."""
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is visible: the makers succeed")
+    for prec in (64, 32):
+        lib = mf.Minfft(prec)
+        assert not lib.mkaux_raw("minfft_mkaux_dft_1d", 8)
+        assert "device" in mf.last_error()
+        assert not lib.mkaux_raw("minfft_mkaux_t2t3_2d", 8, 8)
+    with pytest.raises(mf.MinfftError, match="device"):
+        mf.Batch([("dft", (8,), 0, 0)], mf.PREC_FP32)
