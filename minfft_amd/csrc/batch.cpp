@@ -13,8 +13,10 @@
 // radix-2-split kernel covers it in one).  The short 1D complex groups
 // (2^4 .. 2^12 points) of one direction instead share one persistent launch
 // (multi.hpp).  Each group's launch schedule is picked when the plan is made
-// by timing candidates on the device (autotune.cpp).  All launches are
-// captured once into a HIP graph per (in, out, stream) and replayed.
+// by timing candidates on the device (autotune.cpp).  The launches of a plan
+// with several units of work are captured once into a HIP graph per (in,
+// out, stream), their units on concurrent branches, and replayed; a plan
+// with one unit launches straight into the stream (minfft_batch_run).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
