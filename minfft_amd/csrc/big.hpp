@@ -12,7 +12,9 @@
 // step 2 read it and write the output with neighbouring lanes on
 // neighbouring rows k1 -- coalesced on both sides without a second staging.
 // Work-groups stride over the batch entries (static round robin; entries are
-// independent).
+// independent).  The input is read and the output written with non-temporal
+// accesses: streamed once, they would otherwise push the scratch slots out
+// of the MALL between the two steps (747 x 2^16 FP32: 303 -> 245 us).
 #pragma once
 #include "fft_core.hpp"
 
@@ -66,7 +68,7 @@ __global__ void __launch_bounds__(BT, WPE) big_kernel(BigArgs A) {
             const int n2 = col0 + c1;
             C2 v[E];
 #pragma unroll
-            for (int m = 0; m < E; ++m) v[m] = x[(long long)(tl1 + m * TPT1) * N2 + n2];
+            for (int m = 0; m < E; ++m) v[m] = ld_nt<T>(&x[(long long)(tl1 + m * TPT1) * N2 + n2]);
             C2* slot = lds + c1 * SC1;
             Core1::run(v, slot, tl1, tw);
             // W_N^{n2*k1}, k1 = tl1 + m*TPT1, by an FP64 recurrence over m
@@ -98,7 +100,7 @@ __global__ void __launch_bounds__(BT, WPE) big_kernel(BigArgs A) {
             C2* slot = lds + c2i * SC2;
             Core2::run(v, slot, tl2, tw);
 #pragma unroll
-            for (int m = 0; m < E; ++m) y[(long long)(tl2 + m * TPT2) * N1 + k1] = v[m];
+            for (int m = 0; m < E; ++m) st_nt<T>(&y[(long long)(tl2 + m * TPT2) * N1 + k1], v[m]);
             __syncthreads();
         }
     }

@@ -43,6 +43,27 @@ template <typename T> __device__ __forceinline__ c2<T> cmulc(c2<T> a, c2<T> w) {
     return mk<T>(a.x * w.x + a.y * w.y, a.y * w.x - a.x * w.y);
 }
 // a * (-i) and a * (+i)
+// Non-temporal (streaming) global load / store of one complex value: for
+// data read once and results written once, so that they do not displace
+// data that is read again soon (a fused kernel's scratch slots)
+template <typename T> struct NtVec;
+template <> struct NtVec<float> { typedef float V __attribute__((ext_vector_type(2))); };
+template <> struct NtVec<double> { typedef double V __attribute__((ext_vector_type(2))); };
+template <typename T>
+__device__ __forceinline__ c2<T> ld_nt(const c2<T>* p) {
+    typedef typename NtVec<T>::V V;
+    const V v = __builtin_nontemporal_load(reinterpret_cast<const V*>(p));
+    return mk<T>(v.x, v.y);
+}
+template <typename T>
+__device__ __forceinline__ void st_nt(c2<T>* p, c2<T> a) {
+    typedef typename NtVec<T>::V V;
+    V v;
+    v.x = a.x;
+    v.y = a.y;
+    __builtin_nontemporal_store(v, reinterpret_cast<V*>(p));
+}
+
 template <typename T> __device__ __forceinline__ c2<T> mul_mi(c2<T> a) { return mk<T>(a.y, -a.x); }
 template <typename T> __device__ __forceinline__ c2<T> mul_pi(c2<T> a) { return mk<T>(-a.y, a.x); }
 

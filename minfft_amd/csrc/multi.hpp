@@ -61,12 +61,12 @@ __device__ __forceinline__ void multi_item(const MultiArgs& A, const MultiGroup&
     const C2* __restrict__ x = reinterpret_cast<const C2*>(A.in) + ib;
     C2 v[E];
 #pragma unroll
-    for (int m = 0; m < E; ++m) v[m] = x[tl + m * TPT];
+    for (int m = 0; m < E; ++m) v[m] = ld_nt<T>(&x[tl + m * TPT]);  // read once
     Core<T, LOGL, E, INV>::run(v, lds + c * slot_len(L), tl, reinterpret_cast<const C2*>(A.tw), TPT <= 64);
     if (valid) {
         C2* __restrict__ y = reinterpret_cast<C2*>(A.out) + ob;
 #pragma unroll
-        for (int m = 0; m < E; ++m) y[tl + m * TPT] = v[m];
+        for (int m = 0; m < E; ++m) st_nt<T>(&y[tl + m * TPT], v[m]);  // written once
     }
 }
 

@@ -13,6 +13,9 @@
 // registers, runs DFT_H(a) through the LDS slot while b waits in registers,
 // then DFT_H(b) while DFT_H(a)'s result waits, and writes the pairs
 // (X[2k], X[2k+1]) of its owned k -- adjacent lanes, adjacent pairs.
+// Input and output move with non-temporal accesses (streamed once: in a
+// batch they would otherwise displace the fused kernel's scratch slots of
+// the groups beside them).
 // Contiguous transforms only (item stride 1 on both sides).
 #pragma once
 #include "tile.hpp"
@@ -37,8 +40,8 @@ __global__ void __launch_bounds__((1 << LOGH) / E, WPE) split2_kernel(TileArgs A
     C2 v[E], w[E];
 #pragma unroll
     for (int m = 0; m < E; ++m) {
-        v[m] = in[tl + m * TPT];
-        w[m] = in[tl + m * TPT + H];
+        v[m] = ld_nt<T>(&in[tl + m * TPT]);
+        w[m] = ld_nt<T>(&in[tl + m * TPT + H]);
     }
 #pragma unroll
     for (int m = 0; m < E; ++m) {
@@ -55,8 +58,8 @@ __global__ void __launch_bounds__((1 << LOGH) / E, WPE) split2_kernel(TileArgs A
 #pragma unroll
     for (int m = 0; m < E; ++m) {
         const int k = tl + m * TPT;
-        out[2 * k] = v[m];
-        out[2 * k + 1] = w[m];
+        st_nt<T>(&out[2 * k], v[m]);
+        st_nt<T>(&out[2 * k + 1], w[m]);
     }
 }
 
