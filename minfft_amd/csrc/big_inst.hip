@@ -52,6 +52,10 @@ void register_big(BigTable& t) {
     t.mfma512[1] = reinterpret_cast<const void*>(&mfma512_kernel<true>);
     t.mfma512cf[0] = reinterpret_cast<const void*>(&mfma512_cf_kernel<false>);
     t.mfma512cf[1] = reinterpret_cast<const void*>(&mfma512_cf_kernel<true>);
+    t.mfma512nt[0] = reinterpret_cast<const void*>(&mfma512_kernel<false, true>);
+    t.mfma512nt[1] = reinterpret_cast<const void*>(&mfma512_kernel<true, true>);
+    t.mfma512cfnt[0] = reinterpret_cast<const void*>(&mfma512_cf_kernel<false, true>);
+    t.mfma512cfnt[1] = reinterpret_cast<const void*>(&mfma512_cf_kernel<true, true>);
     // <T, log2 H, min waves per SIMD>: the first length past one
     // work-group's LDS, and the two lengths below it (two work-groups per CU)
     reg_split2<float, 14, 4>(t, 1);

@@ -19,6 +19,8 @@ struct BigTable {
     // 512-point FP32 tiles with the split-FP16 MFMA radix-16 stage (mfma.hpp)
     const void* mfma512[2];
     const void* mfma512cf[2];  // column variant (16 adjacent transforms per work-group)
+    const void* mfma512nt[2];  // the two with non-temporal direct I/O
+    const void* mfma512cfnt[2];
     // single-pass radix-2-split kernels (split2.hpp), by log2 L; variant 1
     // runs the halves with 32 points per lane (radix-32 passes)
     const void* split2[2][17][2][2];  // [prec][log2 L][variant][inverse]

@@ -30,9 +30,12 @@ static void reg_tiles(KTable& t) {
         constexpr int E = default_E(LOGL);
         t.tile[INST_KIND][LOGL] = reinterpret_cast<const void*>(&tile_kernel<IT, LOGL, E, INST_KIND>);
         t.tile_E[LOGL] = E;
-        if constexpr (INST_KIND == K_DFT || INST_KIND == K_INVDFT)
+        if constexpr (INST_KIND == K_DFT || INST_KIND == K_INVDFT) {
             t.tilepk[INST_KIND == K_INVDFT][LOGL] =
                 reinterpret_cast<const void*>(&tile_kernel<IT, LOGL, E, INST_KIND, true>);
+            t.tilent[INST_KIND == K_INVDFT][LOGL] =
+                reinterpret_cast<const void*>(&tile_kernel<IT, LOGL, E, INST_KIND, false, true>);
+        }
         if constexpr ((INST_KIND == K_DFT || INST_KIND == K_INVDFT) && LOGL >= kSmallLogMin &&
                       LOGL <= kSmallLogMax)
             t.tile4[INST_KIND == K_INVDFT][LOGL] = reinterpret_cast<const void*>(&tile_kernel<IT, LOGL, 4, INST_KIND>);

@@ -15,6 +15,8 @@ struct KTable {
     const void* tile32[K_COUNT][kMaxLog + 1];
     // complex tiles with the packed pair (tile_kernel PACK), [inverse][logL]
     const void* tilepk[2][kMaxLog + 1];
+    // complex tiles with non-temporal direct I/O (tile_kernel NT), [inverse][logL]
+    const void* tilent[2][kMaxLog + 1];
     // small-pass complex tiles, 4 points per lane ([inverse][logL], logL
     // kSmallLogMin..kSmallLogMax; null elsewhere)
     const void* tile4[2][kMaxLog + 1];

@@ -35,6 +35,13 @@ typedef float float4_t __attribute__((ext_vector_type(4)));
 typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
 typedef float float2_t __attribute__((ext_vector_type(2)));
 
+// 16-byte non-temporal load (two adjacent complex points)
+__device__ __forceinline__ float4 ld4_nt(const float2* p) {
+    typedef float V4 __attribute__((ext_vector_type(4)));
+    const V4 v = __builtin_nontemporal_load(reinterpret_cast<const V4*>(p));
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+
 __device__ __forceinline__ void split_h(float x, _Float16& hi, _Float16& lo) {
     hi = (_Float16)x;
     lo = (_Float16)(x - (float)hi);
@@ -61,7 +68,8 @@ __device__ __forceinline__ void split4(float a0, float a1, float a2, float a3, h
 // C = A.C transforms per work-group (even, at most 16), 32 lanes each; every
 // wavefront owns two whole transforms (2w, 2w+1), so apart from staged
 // column I/O the work-group's wavefronts never wait on each other.
-template <bool INV>
+// NT: direct I/O with non-temporal accesses (passes larger than the MALL)
+template <bool INV, bool NT = false>
 __global__ void __launch_bounds__(512) mfma512_kernel(TileArgs A) {
     using C2 = float2;
     constexpr int L = 512, E = 16, TPT = L / E;
@@ -107,11 +115,13 @@ __global__ void __launch_bounds__(512) mfma512_kernel(TileArgs A) {
             } else if ((reinterpret_cast<unsigned long long>(row) & 15) == 0) {
 #pragma unroll
                 for (int q = 0; q < 4; ++q)
-                    P[tt][q] = *static_cast<const float4*>(__builtin_assume_aligned(src + 32 * q, 16));
+                    P[tt][q] = NT ? ld4_nt(src + 32 * q)
+                                  : *static_cast<const float4*>(__builtin_assume_aligned(src + 32 * q, 16));
             } else {
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
-                    const C2 a = src[32 * q], b = src[32 * q + 1];
+                    const C2 a = NT ? ld_nt<float>(src + 32 * q) : src[32 * q];
+                    const C2 b = NT ? ld_nt<float>(src + 32 * q + 1) : src[32 * q + 1];
                     P[tt][q] = make_float4(a.x, a.y, b.x, b.y);
                 }
             }
@@ -224,7 +234,10 @@ __global__ void __launch_bounds__(512) mfma512_kernel(TileArgs A) {
         if (me.valid) {
             const Walk<C2> wo(reinterpret_cast<C2*>(A.out) + me.ob, 0, 1, tl, TPT, A.out_e);
 #pragma unroll
-            for (int m = 0; m < E; ++m) wo[m] = v[m];
+            for (int m = 0; m < E; ++m) {
+                if constexpr (NT) st_nt<float>(&wo[m], v[m]);
+                else wo[m] = v[m];
+            }
         }
         return;
     }
@@ -251,7 +264,7 @@ __global__ void __launch_bounds__(512) mfma512_kernel(TileArgs A) {
 // butterfly j): its 16 inputs sit in lanes c + 16g.  Passes 2 and 3 run
 // with the c-fast core mapping (lane t: transform t % 16, thread t / 16),
 // whose owned positions give coalesced direct stores.
-template <bool INV>
+template <bool INV, bool NT = false>
 __global__ void __launch_bounds__(512) mfma512_cf_kernel(TileArgs A) {
     using C2 = float2;
     constexpr int L = 512, E = 16, TPT = L / E, CT = 16;
@@ -280,7 +293,8 @@ __global__ void __launch_bounds__(512) mfma512_cf_kernel(TileArgs A) {
 #pragma unroll
         for (int i = 0; i < 4; ++i)
 #pragma unroll
-            for (int q = 0; q < 4; ++q) x[i][q] = p0[(long long)(8 * i + 32 * q) * es];
+            for (int q = 0; q < 4; ++q)
+                x[i][q] = NT ? ld_nt<float>(&p0[(long long)(8 * i + 32 * q) * es]) : p0[(long long)(8 * i + 32 * q) * es];
     } else {
 #pragma unroll
         for (int i = 0; i < 4; ++i)
@@ -354,7 +368,10 @@ __global__ void __launch_bounds__(512) mfma512_cf_kernel(TileArgs A) {
     if (me.valid) {
         const Walk<C2> wo(reinterpret_cast<C2*>(A.out) + me.ob, 0, 1, tl, TPT, A.out_e);
 #pragma unroll
-        for (int mm = 0; mm < E; ++mm) wo[mm] = v[mm];
+        for (int mm = 0; mm < E; ++mm) {
+            if constexpr (NT) st_nt<float>(&wo[mm], v[mm]);
+            else wo[mm] = v[mm];
+        }
     }
 }
 

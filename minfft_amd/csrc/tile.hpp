@@ -553,8 +553,10 @@ template <typename T> constexpr int max_threads() { return sizeof(T) == 8 ? 512 
 template <typename T, int E> constexpr int tile_threads() { return E >= 32 ? 512 : max_threads<T>(); }
 
 // PACK (complex pass over a real DFT's half spectrum, direct c-fast I/O on
-// both sides): see `packed` below.
-template <typename T, int LOGL, int E, int KIND, bool PACK = false>
+// both sides): see `packed` below.  NT: the direct complex I/O uses
+// non-temporal accesses (passes over arrays larger than the MALL, which
+// stream through it anyway).
+template <typename T, int LOGL, int E, int KIND, bool PACK = false, bool NT = false>
 __global__ void __launch_bounds__((tile_threads<T, E>())) tile_kernel(TileArgs A) {
     using C2 = c2<T>;
     constexpr int L = 1 << LOGL;
@@ -614,13 +616,13 @@ __global__ void __launch_bounds__((tile_threads<T, E>())) tile_kernel(TileArgs A
                 for (int m = 0; m < E; ++m) v[m] = mk<T>(ra[m * step], rb[m * step]);
             } else if (A.in_e == 1) {
 #pragma unroll
-                for (int m = 0; m < E; ++m) v[m] = in[tl + m * TPT];
+                for (int m = 0; m < E; ++m) v[m] = NT ? ld_nt<T>(&in[tl + m * TPT]) : in[tl + m * TPT];
             } else {
                 const long long es = A.in_e;
                 const C2* __restrict__ b = in + (long long)tl * es;
                 const long long step = es * TPT;
 #pragma unroll
-                for (int m = 0; m < E; ++m) v[m] = b[m * step];
+                for (int m = 0; m < E; ++m) v[m] = NT ? ld_nt<T>(&b[m * step]) : b[m * step];
             }
         } else {
 #pragma unroll
@@ -750,13 +752,19 @@ __global__ void __launch_bounds__((tile_threads<T, E>())) tile_kernel(TileArgs A
             if (me.valid) {
                 if (A.out_e == 1) {
 #pragma unroll
-                    for (int m = 0; m < E; ++m) out[tl + m * TPT] = v[m];
+                    for (int m = 0; m < E; ++m) {
+                        if constexpr (NT) st_nt<T>(&out[tl + m * TPT], v[m]);
+                        else out[tl + m * TPT] = v[m];
+                    }
                 } else {
                     const long long es = A.out_e;
                     C2* __restrict__ b = out + (long long)tl * es;
                     const long long step = es * TPT;
 #pragma unroll
-                    for (int m = 0; m < E; ++m) b[m * step] = v[m];
+                    for (int m = 0; m < E; ++m) {
+                        if constexpr (NT) st_nt<T>(&b[m * step], v[m]);
+                        else b[m * step] = v[m];
+                    }
                 }
             }
             return;
