@@ -386,16 +386,13 @@ static int launch_tile(Prec prec, int kind, int logL, const void* tw, const void
     // transforms per wavefront, direct I/O for contiguous rows, other I/O
     // staged through LDS)
     const bool s16cf = split16 && in_cf && out_cf && cf_core;
-    // passes over more data than the MALL holds: non-temporal direct I/O
-    // (see the complex tiles below)
-    const bool stream = ntr * (unsigned long long)L * (prec == PREC_F64 ? 16u : 8u) >= kStreamBytes;
     if (s16cf) {
-        fn = (stream ? bigtable().mfma512cfnt : bigtable().mfma512cf)[kind == K_INVDFT];
+        fn = bigtable().mfma512cf[kind == K_INVDFT];
         A.cfast = 1;
         A.in_mode = IO_DIRECT;
         A.out_mode = IO_DIRECT;
     } else if (split16) {
-        fn = (stream ? bigtable().mfma512nt : bigtable().mfma512)[kind == K_INVDFT];
+        fn = bigtable().mfma512[kind == K_INVDFT];
         A.cfast = 0;
         A.in_mode = in_row ? IO_DIRECT : (in_cf ? IO_STAGE_CF : IO_STAGE_ROW);
         A.out_mode = out_row ? IO_DIRECT : (out_cf ? IO_STAGE_CF : IO_STAGE_ROW);
@@ -414,14 +411,6 @@ static int launch_tile(Prec prec, int kind, int logL, const void* tw, const void
         A.lgB[3] = (A.B[3] & (A.B[3] - 1)) == 0 ? ilog2i((int)A.B[3]) : -1;
         A.ntrans = (unsigned)ntr;
     }
-    // a complex pass over more data than the MALL holds streams through it
-    // anyway: non-temporal direct I/O keeps it from displacing what is
-    // re-read (512^3 FP32: 1.30 -> 1.24 ms); smaller passes keep the
-    // MALL's reuse between passes (2048^2 FP64 passes were 12-24 % slower
-    // non-temporal)
-    if ((kind == K_DFT || kind == K_INVDFT) && fn == kt.tile[kind][logL] && kt.tilent[kind == K_INVDFT][logL] &&
-        stream)
-        fn = kt.tilent[kind == K_INVDFT][logL];
     const bool wants_cf = A.cfast || A.in_mode == IO_STAGE_CF || A.out_mode == IO_STAGE_CF;
     const size_t cs = prec == PREC_F64 ? 16 : 8;
     const int sc = slot_len(L);
@@ -451,6 +440,21 @@ static int launch_tile(Prec prec, int kind, int logL, const void* tw, const void
     while (C & (C - 1)) C &= C - 1;
     if (split16 && C < 2) C = 2;  // MFMA operands span a full wavefront: two transforms
     if (s16cf) C = 16;            // the column variant's layout: 16 transforms, 512 lanes
+    // A complex pass over more data than the MALL holds streams through it
+    // anyway: non-temporal direct I/O keeps it from displacing what is
+    // re-read (512^3 FP32: 1.30 -> 1.24 ms).  Smaller passes keep the MALL's
+    // reuse between passes (2048^2 FP64 passes were 12-24 % slower
+    // non-temporal), and so do column tiles whose row segments are shorter
+    // than a 128-byte line: a non-temporal access does not keep the rest of
+    // the line for the neighbouring tile (32-byte segments: 4x slower).
+    const bool stream = ntr * (unsigned long long)L * cs >= kStreamBytes && (!wants_cf || (size_t)C * cs >= 128);
+    if (stream) {
+        const BigTable& bt = bigtable();
+        if (fn == bt.mfma512cf[kind == K_INVDFT]) fn = bt.mfma512cfnt[kind == K_INVDFT];
+        else if (fn == bt.mfma512[kind == K_INVDFT]) fn = bt.mfma512nt[kind == K_INVDFT];
+        else if ((kind == K_DFT || kind == K_INVDFT) && fn == kt.tile[kind][logL] && kt.tilent[kind == K_INVDFT][logL])
+            fn = kt.tilent[kind == K_INVDFT][logL];
+    }
     A.C = C;
     A.lgC = ilog2i(C);
     A.sc = sc;
