@@ -34,8 +34,10 @@ static void reg_one(BigTable& t, int prec, int variant = 0) {
 template <typename T, int LOGH, int WPE, int E = 16>
 static void reg_split2(BigTable& t, int prec) {
     constexpr int v = E == 32 ? 1 : 0;
-    t.split2[prec][LOGH + 1][v][0] = reinterpret_cast<const void*>(&split2_kernel<T, LOGH, E, false, WPE>);
-    t.split2[prec][LOGH + 1][v][1] = reinterpret_cast<const void*>(&split2_kernel<T, LOGH, E, true, WPE>);
+    t.split2[prec][LOGH + 1][v][0] = reinterpret_cast<const void*>(&split2_kernel<T, LOGH, E, false, WPE, false>);
+    t.split2[prec][LOGH + 1][v][1] = reinterpret_cast<const void*>(&split2_kernel<T, LOGH, E, true, WPE, false>);
+    t.split2nt[prec][LOGH + 1][v][0] = reinterpret_cast<const void*>(&split2_kernel<T, LOGH, E, false, WPE, true>);
+    t.split2nt[prec][LOGH + 1][v][1] = reinterpret_cast<const void*>(&split2_kernel<T, LOGH, E, true, WPE, true>);
     t.split2_bt[prec][LOGH + 1][v] = (1 << LOGH) / E;
     t.split2_lds[prec][LOGH + 1] = split2_lds_bytes<T, LOGH>();
 }

@@ -24,6 +24,7 @@ struct BigTable {
     // single-pass radix-2-split kernels (split2.hpp), by log2 L; variant 1
     // runs the halves with 32 points per lane (radix-32 passes)
     const void* split2[2][17][2][2];  // [prec][log2 L][variant][inverse]
+    const void* split2nt[2][17][2][2];  // the same with non-temporal I/O
     int split2_bt[2][17][2];
     size_t split2_lds[2][17];
     // residue-split single-pass kernels (rsplit.hpp), by log2 N, log2 P - 1

@@ -524,8 +524,11 @@ static int launch_split2(const Workspace& ws, Prec prec, int kind, int logL, con
     A.ntiles = (unsigned)ntr;
     A.C = 1;
     A.xcd = 1;
+    // non-temporal I/O for whole 1D transforms and passes larger than the MALL
+    const bool nt = lv.stream_io || ntr * (1ull << logL) * (prec == PREC_F64 ? 16u : 8u) >= kStreamBytes;
     void* args[] = {&A};
-    return launch_fn(bt.split2[pi][logL][v][kind == K_INVDFT], (unsigned)ntr, (unsigned)bt.split2_bt[pi][logL][v],
+    return launch_fn((nt ? bt.split2nt : bt.split2)[pi][logL][v][kind == K_INVDFT], (unsigned)ntr,
+                     (unsigned)bt.split2_bt[pi][logL][v],
                      args, bt.split2_lds[pi][logL], st);
 }
 
@@ -1030,6 +1033,7 @@ int run_entries(const Workspace& ws, Prec pr, int kind, int d, const int* Ns, co
             Levels ax = axis_levels(Dm, a, S);
             const bool first = (a == d - 1);
             Levels lv = mk(ax, first ? ioff : ooff, ooff);
+            lv.stream_io = (d == 1);
             int rc = axis_pass(ws, pr, kind, ilog2i(Dm[a]), first ? in : (const void*)out, S, out, S, lv, st);
             if (rc) return rc;
         }

@@ -33,6 +33,9 @@ struct Levels {
     // (both columns real), or the inverse pass right before it (only their
     // real parts are read)
     bool pack = false;
+    // the pass is a whole 1D transform: its input is read once and its output
+    // written once (non-temporal I/O where the kernel has it)
+    bool stream_io = false;
     void push(unsigned b, long long i, long long o) {
         B[n] = b;
         is[n] = i;

@@ -13,9 +13,11 @@
 // registers, runs DFT_H(a) through the LDS slot while b waits in registers,
 // then DFT_H(b) while DFT_H(a)'s result waits, and writes the pairs
 // (X[2k], X[2k+1]) of its owned k -- adjacent lanes, adjacent pairs.
-// Input and output move with non-temporal accesses (streamed once: in a
-// batch they would otherwise displace the fused kernel's scratch slots of
-// the groups beside them).
+// NT: input and output move with non-temporal accesses -- for whole 1D
+// transforms (streamed once: in a batch they would otherwise displace the
+// fused kernel's scratch slots of the groups beside them) and passes larger
+// than the MALL; a row pass of a smaller multi-dimensional array keeps its
+// output in the MALL for the next pass.
 // Contiguous transforms only (item stride 1 on both sides).
 #pragma once
 #include "tile.hpp"
@@ -25,7 +27,7 @@ namespace mfx {
 // E: points per lane of the half-length FFTs (16, or 32 for radix-32
 // passes); WPE: minimum waves per SIMD (register budget): 4 lets two
 // 512-lane work-groups share a CU when their LDS fits twice
-template <typename T, int LOGH, int E, bool INV, int WPE>
+template <typename T, int LOGH, int E, bool INV, int WPE, bool NT>
 __global__ void __launch_bounds__((1 << LOGH) / E, WPE) split2_kernel(TileArgs A) {
     using C2 = c2<T>;
     constexpr int H = 1 << LOGH, L = 2 * H, TPT = H / E;
@@ -40,8 +42,8 @@ __global__ void __launch_bounds__((1 << LOGH) / E, WPE) split2_kernel(TileArgs A
     C2 v[E], w[E];
 #pragma unroll
     for (int m = 0; m < E; ++m) {
-        v[m] = ld_nt<T>(&in[tl + m * TPT]);
-        w[m] = ld_nt<T>(&in[tl + m * TPT + H]);
+        v[m] = NT ? ld_nt<T>(&in[tl + m * TPT]) : in[tl + m * TPT];
+        w[m] = NT ? ld_nt<T>(&in[tl + m * TPT + H]) : in[tl + m * TPT + H];
     }
 #pragma unroll
     for (int m = 0; m < E; ++m) {
@@ -58,8 +60,13 @@ __global__ void __launch_bounds__((1 << LOGH) / E, WPE) split2_kernel(TileArgs A
 #pragma unroll
     for (int m = 0; m < E; ++m) {
         const int k = tl + m * TPT;
-        st_nt<T>(&out[2 * k], v[m]);
-        st_nt<T>(&out[2 * k + 1], w[m]);
+        if constexpr (NT) {
+            st_nt<T>(&out[2 * k], v[m]);
+            st_nt<T>(&out[2 * k + 1], w[m]);
+        } else {
+            out[2 * k] = v[m];
+            out[2 * k + 1] = w[m];
+        }
     }
 }
 
