@@ -351,7 +351,10 @@ static int enqueue_batch(minfft_batch_plan* p, const void* in, void* out, hipStr
     std::vector<std::pair<long long, int>> units;
     for (int gi = 0; gi < (int)p->groups.size(); ++gi)
         if (!p->groups[gi].in_multi) units.push_back({group_bytes(p->groups[gi], p->prec), gi});
-    for (int k = 0; k < (int)p->multi.size(); ++k) units.push_back({p->multi[k].bytes, -1 - k});
+    // (a persistent launch counts three times its bytes: on the default batch
+    // it then gets a branch of its own and starts early instead of trailing
+    // the 2^13 group, 0.424 against 0.428 ms per step over six A/B runs)
+    for (int k = 0; k < (int)p->multi.size(); ++k) units.push_back({p->multi[k].bytes * 3, -1 - k});
     std::stable_sort(units.begin(), units.end(), [](const std::pair<long long, int>& a,
                                                     const std::pair<long long, int>& b) { return a.first > b.first; });
     auto run_unit = [&](int u, hipStream_t s) -> int {
@@ -400,7 +403,7 @@ static int enqueue_batch(minfft_batch_plan* p, const void* in, void* out, hipStr
             // branch 0 (1D complex groups whose schedule is single-pass do not)
             const BatchGroup& g = p->groups[u.second];
             const bool single = g.d == 1 && (g.kind == K_DFT || g.kind == K_INVDFT) &&
-                                c2c_single_pass(p->prec, lg2(g.n[0]), g.tune.sched);
+                                c2c_single_pass(p->prec, lg2(g.n[0]), g.tune.sched, in == out);
             if (g.fbytes && !single) b = 0;
         }
         load[b] += u.first;

@@ -540,6 +540,9 @@ static int launch_rsplit(const Workspace& ws, Prec prec, int kind, int logL, con
     const int code = tune().rsplit;
     const int lp = code / 10, b = code % 10;
     if (code <= 0 || lp < 1 || lp > 4 || b < 1 || b > 2 || in_e != 1 || out_e != 1 || logL > 16) return 1;
+    // the P/B work-groups of a transform all read its whole input while
+    // they write their outputs: never in place
+    if (in == out) return 1;
     const BigTable& bt = bigtable();
     const int pi = prec == PREC_F64 ? 0 : 1;
     const void* fn = bt.rsplit[pi][logL][lp - 1][b - 1][kind == K_INVDFT];
@@ -606,7 +609,7 @@ std::vector<int> rsplit_codes(Prec prec, int logN) {
     return v;
 }
 
-bool c2c_single_pass(Prec prec, int logL, const Tune& t) {
+bool c2c_single_pass(Prec prec, int logL, const Tune& t, bool in_place) {
     if (!ensure_device()) return false;
     const int maxlog = device().maxlog[prec];
     if (logL <= maxlog) return true;
@@ -615,8 +618,8 @@ bool c2c_single_pass(Prec prec, int logL, const Tune& t) {
     const int pi = prec == PREC_F64 ? 0 : 1;
     const size_t lmax = (size_t)device().lds_max;
     const int lp = t.rsplit / 10, b = t.rsplit % 10;
-    if (t.rsplit > 0 && lp >= 1 && lp <= 4 && b >= 1 && b <= 2 && bt.rsplit[pi][logL][lp - 1][b - 1][0] &&
-        bt.rsplit_lds[pi][logL][lp - 1][b - 1] <= lmax)
+    if (!in_place && t.rsplit > 0 && lp >= 1 && lp <= 4 && b >= 1 && b <= 2 &&
+        bt.rsplit[pi][logL][lp - 1][b - 1][0] && bt.rsplit_lds[pi][logL][lp - 1][b - 1] <= lmax)
         return true;
     const int sp2 = t.split2 >= 0 ? t.split2 : (prec == PREC_F64 ? 2 : 3);
     return sp2 >= 1 && logL == maxlog + 1 && bt.split2[pi][logL][0][0] && bt.split2_lds[pi][logL] <= lmax;

@@ -181,8 +181,9 @@ int launch_multi(Prec prec, bool inv, const MultiArgs& a, unsigned grid, hipStre
 std::vector<int> rsplit_codes(Prec prec, int logN);
 // true when a contiguous complex pass of length 2^logL runs as one
 // single-pass launch under schedule t (tile, residue-split or radix-2-split
-// kernel), i.e. touches no two-step scratch
-bool c2c_single_pass(Prec prec, int logL, const Tune& t);
+// kernel), i.e. touches no two-step scratch; the residue-split kernel runs
+// only out of place
+bool c2c_single_pass(Prec prec, int logL, const Tune& t, bool in_place = false);
 // scratch (bytes) an axis pass of this shape needs in ws.fbuf / ws.gbuf;
 // in_extent = number of input items addressed (for the mirrored layout)
 size_t axis_scratch_bytes(Prec prec, int kind, int logN, unsigned long long ntrans,

@@ -238,3 +238,43 @@ def test_split_fp16_schedules(oracle64):
             e = rel_l2(y, refs[i])
             assert e <= TOL32, (spec, i, kinds[i], shapes[i], e)
     batch.close()
+
+
+@pytest.mark.parametrize("prec", [32, 64])
+def test_residue_split_never_in_place(oracle64, prec):
+    """The residue-split kernel's work-groups all read a transform's whole
+    input while writing their outputs, so an in-place run (input and output
+    the same buffer) with the residue-split schedule pinned takes another
+    single-pass or two-step kernel: the results still match the oracle, and
+    the groups beside it (sharing the two-step scratch) are not disturbed."""
+    import torch
+    n = 1 << 15
+    entries = [("dft", (n,))] * 6 + [("realdft", (1 << 17,))] * 2 + [("invdft", (1 << 14,))] * 4
+    xs = [make_input(k, s, prec, 950 + i) for i, (k, s) in enumerate(entries)]
+    cs = 16 if prec == 64 else 8
+    offs, pos = [], 0
+    for (k, s), x in zip(entries, xs):
+        offs.append(pos)
+        pos += max(x.nbytes, int(np.prod(out_shape(k, s))) * (cs if is_complex_out(k) else cs // 2))
+        pos = (pos + 255) & ~255
+    buf = np.zeros(pos, dtype=np.uint8)
+    for x, o in zip(xs, offs):
+        buf[o:o + x.nbytes] = np.frombuffer(np.ascontiguousarray(x).tobytes(), dtype=np.uint8)
+    d = torch.from_numpy(buf).cuda()
+    b = mf.Batch([(k, s, o, o) for (k, s), o in zip(entries, offs)], mf.PREC_FP64 if prec == 64 else mf.PREC_FP32)
+    ngroups = b.describe().count("\n")
+    for g in range(ngroups):
+        b.schedule(g, "rsplit=22" if prec == 32 else "rsplit=31")
+    print(f"[case] in-place residue-split schedule prec={prec}", flush=True)
+    b.run(d.data_ptr(), d.data_ptr(), None)
+    torch.cuda.synchronize()
+    out = d.cpu().numpy()
+    b.close()
+    bar = lambda s: tol64(s) if prec == 64 else TOL32
+    for i, ((k, s), x) in enumerate(zip(entries, xs)):
+        odt = (np.complex128 if prec == 64 else np.complex64) if is_complex_out(k) else \
+            (np.float64 if prec == 64 else np.float32)
+        nout = int(np.prod(out_shape(k, s)))
+        y = out[offs[i]:offs[i] + nout * np.dtype(odt).itemsize].view(odt).reshape(out_shape(k, s))
+        ref = oracle64.run(k, x.astype(np.complex128 if is_complex_in(k) else np.float64), s)
+        assert rel_l2(y, ref) <= bar(s), (i, k, s, rel_l2(y, ref))
