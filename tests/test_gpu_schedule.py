@@ -278,3 +278,40 @@ def test_residue_split_never_in_place(oracle64, prec):
         y = out[offs[i]:offs[i] + nout * np.dtype(odt).itemsize].view(odt).reshape(out_shape(k, s))
         ref = oracle64.run(k, x.astype(np.complex128 if is_complex_in(k) else np.float64), s)
         assert rel_l2(y, ref) <= bar(s), (i, k, s, rel_l2(y, ref))
+
+
+def test_plain_in_place_with_residue_split_schedule(tmp_path):
+    """A plain plan whose process-wide schedule asks for the residue-split
+    kernel (MINFFT_TUNE, read once per process: a fresh process) runs out
+    of place on device arrays through the residue-split kernel, and in
+    place (device arrays, or host arrays, which are transformed in place in
+    the device staging buffer) through the other single-pass kernel; all
+    match numpy."""
+    import os
+    import subprocess
+    import sys
+    script = tmp_path / "inplace.py"
+    script.write_text(
+        "import sys, numpy as np\n"
+        f"sys.path.insert(0, {os.path.dirname(os.path.dirname(os.path.abspath(__file__)))!r})\n"
+        "import minfft_amd as mf\n"
+        "n = 1 << 15\n"
+        "rng = np.random.default_rng(5)\n"
+        "x = (rng.uniform(-1, 1, n) + 1j * rng.uniform(-1, 1, n)).astype(np.complex64)\n"
+        "ref = np.fft.fft(x.astype(np.complex128))\n"
+        "import torch\n"
+        "aux = mf.Aux(mf.Minfft(32), 'dft', (n,))\n"
+        "xd = torch.from_numpy(x).cuda(); yd = torch.empty_like(xd); zd = xd.clone()\n"
+        "aux('dft', xd, yd)\n"
+        "aux('dft', zd, zd)\n"
+        "h = x.copy()\n"
+        "aux('dft', h, h)\n"
+        "for name, r in (('out of place', yd.cpu().numpy()), ('in place', zd.cpu().numpy()), ('host', h)):\n"
+        "    e = np.linalg.norm(r - ref) / np.linalg.norm(ref)\n"
+        "    print(name, e)\n"
+        "    assert e < 1e-6, (name, e)\n"
+        "print('ok')\n")
+    env = dict(os.environ, MINFFT_TUNE="rsplit=22")
+    r = subprocess.run([sys.executable, str(script)], capture_output=True, text=True, timeout=180, env=env)
+    print(r.stdout, r.stderr[-2000:])
+    assert r.returncode == 0 and r.stdout.strip().endswith("ok")
