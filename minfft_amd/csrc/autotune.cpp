@@ -12,7 +12,8 @@
 // knobs that apply to the group one at a time (coordinate descent, up to
 // three rounds), keeping a value when it is at least 3 % faster.  Results are cached per (precision,
 // kind, shape, count bucket) for the process, and in the file named by
-// MINFFT_TUNE_CACHE if set.  MINFFT_AUTOTUNE=0, or a MINFFT_TUNE override,
+// MINFFT_TUNE_CACHE if set (valid for the device and library build named in
+// its header line).  MINFFT_AUTOTUNE=0, or a MINFFT_TUNE override,
 // turns the tuner off (the process-wide schedule is used).
 #include <hip/hip_runtime.h>
 
@@ -27,6 +28,8 @@
 #include <sstream>
 #include <string>
 #include <vector>
+
+#include <unistd.h>
 
 #include "autotune.hpp"
 
@@ -53,7 +56,33 @@ std::string key_of(Prec prec, int kind, int d, const int* n, unsigned count) {
     return b;
 }
 
-// cache file line: key tpw cfb cfcore wave xcd e32 split split2 fused wpc falt rsplit us_default us_best
+// Cache file: a header line naming the device, its CU count and the
+// library build, then one line per group:
+//   key tpw cfb cfcore wave xcd e32 e4 split split2 fused wpc falt rsplit us_default us_best
+// A file whose header names another device or build is ignored (and
+// replaced on the next save).  Saves rewrite the whole file into a
+// temporary one and rename it into place, so a reader never sees a partial
+// file; processes saving at the same time keep the last writer's entries.
+std::string cache_header() {
+    hipDeviceProp_t prop;
+    std::string arch = "unknown";
+    int ncu = 0;
+    if (hipGetDeviceProperties(&prop, device().id) == hipSuccess) {
+        arch = prop.gcnArchName;
+        ncu = prop.multiProcessorCount;
+    }
+    for (char& c : arch)
+        if (c == ' ') c = '_';
+    char b[256];
+    std::snprintf(b, sizeof(b), "%s_%s", __DATE__, __TIME__);
+    // (__DATE__ pads single-digit days with a space)
+    std::string build;
+    for (const char* c = b; *c; ++c)
+        if (*c != ' ') build += *c;
+    std::snprintf(b, sizeof(b), "#minfft-tune v2 arch=%s cus=%d build=%s", arch.c_str(), ncu, build.c_str());
+    return b;
+}
+
 void load_cache() {
     if (g_loaded) return;
     g_loaded = true;
@@ -61,32 +90,45 @@ void load_cache() {
     if (!f) return;
     std::ifstream in(f);
     std::string line;
+    if (!std::getline(in, line) || line != cache_header()) return;
     while (std::getline(in, line)) {
         std::istringstream ss(line);
         std::string k;
         TuneResult r;
         r.sched = base_tune();
         Tune& t = r.sched;
-        if (ss >> k >> t.tpw >> t.cfb >> t.cfcore >> t.wave >> t.xcd >> t.e32 >> t.split >> t.split2 >> t.fused >>
-            t.fused_wpc >> t.fused_alt >> t.rsplit >>
-            r.us_default >> r.us_best) {
+        if (ss >> k >> t.tpw >> t.cfb >> t.cfcore >> t.wave >> t.xcd >> t.e32 >> t.e4 >> t.split >> t.split2 >>
+            t.fused >> t.fused_wpc >> t.fused_alt >> t.rsplit >> r.us_default >> r.us_best) {
             r.tuned = true;
+            r.cached = true;
             g_cache[k] = r;
         }
     }
 }
 
-void save_entry(const std::string& k, const TuneResult& r) {
+// rewrites the cache file with every entry of this process (loaded or tuned)
+void save_cache() {
     const char* f = std::getenv("MINFFT_TUNE_CACHE");
-    if (!f) return;
-    std::ofstream out(f, std::ios::app);
-    const Tune& t = r.sched;
-    out << k << ' ' << t.tpw << ' ' << t.cfb << ' ' << t.cfcore << ' ' << t.wave << ' ' << t.xcd << ' ' << t.e32
-        << ' ' << t.split
-        << ' ' << t.split2
-        << ' ' << t.fused
-        << ' ' << t.fused_wpc << ' ' << t.fused_alt << ' ' << t.rsplit << ' ' << r.us_default << ' ' << r.us_best
-        << '\n';
+    if (!f || !*f) return;
+    const std::string tmp = std::string(f) + ".tmp." + std::to_string((long long)getpid());
+    {
+        std::ofstream out(tmp, std::ios::trunc);
+        if (!out) return;
+        out << cache_header() << '\n';
+        for (const auto& kv : g_cache) {
+            const TuneResult& r = kv.second;
+            const Tune& t = r.sched;
+            out << kv.first << ' ' << t.tpw << ' ' << t.cfb << ' ' << t.cfcore << ' ' << t.wave << ' ' << t.xcd << ' '
+                << t.e32 << ' ' << t.e4 << ' ' << t.split << ' ' << t.split2 << ' ' << t.fused << ' ' << t.fused_wpc
+                << ' ' << t.fused_alt << ' ' << t.rsplit << ' ' << r.us_default << ' ' << r.us_best << '\n';
+        }
+        if (!out.good()) {
+            out.close();
+            std::remove(tmp.c_str());
+            return;
+        }
+    }
+    if (std::rename(tmp.c_str(), f) != 0) std::remove(tmp.c_str());
 }
 
 struct Knob {
@@ -262,7 +304,7 @@ TuneResult autotune_group(const Workspace& ws, const TuneRequest& r, hipStream_t
     if (res.tuned) {
         std::lock_guard<std::mutex> lk(g_mu);
         g_cache[key] = res;
-        save_entry(key, res);
+        save_cache();
     }
     return res;
 }

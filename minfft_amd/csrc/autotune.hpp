@@ -28,6 +28,7 @@ struct TuneResult {
     bool tuned = false;       // measured (else the process-wide schedule)
     float us_default = 0.f;   // subset time with the default schedule
     float us_best = 0.f;      // subset time with `sched`
+    bool cached = false;      // read from the MINFFT_TUNE_CACHE file, not measured here
 };
 
 bool autotune_enabled();

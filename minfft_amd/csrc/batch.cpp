@@ -23,6 +23,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <string>
 #include <tuple>
 #include <utility>
 #include <vector>
@@ -83,6 +84,11 @@ struct minfft_batch_plan {
     // branch streams/events for concurrent groups inside the captured graph
     std::vector<hipStream_t> side;
     std::vector<hipEvent_t> ev;
+    // bytes of `in` / `out` the entries address (largest offset + extent)
+    size_t in_extent = 0, out_extent = 0;
+    // the last (in, out) pair found to be device memory
+    const void* ok_in = nullptr;
+    const void* ok_out = nullptr;
 };
 
 // short 1D complex groups share one persistent launch per direction
@@ -134,6 +140,12 @@ int minfft_enqueue(int kind, const void* in, void* out, const void* aux, void* s
     }
     if (kind < 0 || kind >= K_COUNT) {
         set_error("invalid kind");
+        return -1;
+    }
+    // the kernels dereference in/out directly: host memory would fault the
+    // GPU (no XNACK), so it is refused here
+    if (!is_device_pointer(in) || !is_device_pointer(out)) {
+        set_error("minfft_enqueue needs device (or managed) pointers for in and out");
         return -1;
     }
     hipStream_t st = stream ? reinterpret_cast<hipStream_t>(stream) : p->stream;
@@ -215,7 +227,11 @@ minfft_batch_plan* minfft_mkaux_batch(int count, const minfft_batch_entry* e, in
         BatchGroup& g = p->groups[gi];
         g.ioff.push_back(x.in_offset / (long long)ib);
         g.ooff.push_back(x.out_offset / (long long)ob);
-        (void)P;
+        long long ii = P, oi = P;
+        if (x.kind == K_REALDFT) oi = P / n[x.d - 1] * (n[x.d - 1] / 2 + 1);
+        if (x.kind == K_INVREALDFT) ii = P / n[x.d - 1] * (n[x.d - 1] / 2 + 1);
+        p->in_extent = std::max(p->in_extent, (size_t)x.in_offset + (size_t)ii * ib);
+        p->out_extent = std::max(p->out_extent, (size_t)x.out_offset + (size_t)oi * ob);
     }
     p->ws = Workspace();
     if (hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess) return fail("hipStreamCreate failed");
@@ -423,6 +439,16 @@ int minfft_batch_run(minfft_batch_plan* p, const void* in, void* out, void* stre
         set_error("invalid batch plan");
         return -1;
     }
+    // the kernels dereference in/out directly (host memory would fault the
+    // GPU): checked once per (in, out) pair, i.e. when a graph is captured
+    if (!(in == p->ok_in && out == p->ok_out)) {
+        if (!is_device_pointer(in) || !is_device_pointer(out)) {
+            set_error("minfft_batch_run needs device (or managed) pointers for in and out");
+            return -1;
+        }
+        p->ok_in = in;
+        p->ok_out = out;
+    }
     hipStream_t st = stream ? reinterpret_cast<hipStream_t>(stream) : p->stream;
     // a plan with one unit of work (one group, or only the persistent
     // launch) has no branches to run concurrently: its launches go straight
@@ -513,7 +539,7 @@ int minfft_batch_describe(const minfft_batch_plan* p, char* buf, int len) {
                           kname[g.kind], dims.c_str(), g.ioff.size(), kname[g.kind]);
         else
             std::snprintf(line, sizeof(line), "group %zu: %s %s count=%zu %s: %s default_us=%.1f tuned_us=%.1f\n",
-                          i, kname[g.kind], dims.c_str(), g.ioff.size(), g.tune.tuned ? "tuned" : "untuned",
+                          i, kname[g.kind], dims.c_str(), g.ioff.size(), g.tune.tuned ? (g.tune.cached ? "tuned (cache file)" : "tuned") : "untuned",
                           describe_tune(g.tune.sched).c_str(), g.tune.us_default, g.tune.us_best);
         s += line;
     }
@@ -529,6 +555,16 @@ int minfft_batch(minfft_batch_plan* p, const void* in, size_t in_bytes, void* ou
     clear_error();
     if (!p) {
         set_error("invalid batch plan");
+        return -1;
+    }
+    // the entries must lie inside the buffers: refused before any copy or launch
+    if (in_bytes < p->in_extent || out_bytes < p->out_extent) {
+        set_error("minfft_batch: buffer smaller than the entries address (in needs " +
+                  std::to_string(p->in_extent) + " bytes, out " + std::to_string(p->out_extent) + ")");
+        return -1;
+    }
+    if (!in || !out) {
+        set_error("minfft_batch: null buffer");
         return -1;
     }
     const bool din = is_device_pointer(in), dout = is_device_pointer(out);

@@ -508,3 +508,57 @@ def test_distinct_plans_run_concurrently(oracle64):
             ref = oracle64.run(k, x.astype(np.complex128 if is_complex_in(k) else np.float64), s)
             assert rel_l2(y, ref) <= TOL32, (i, k, s)
         p["b"].close()
+
+
+def test_batch_refuses_short_buffers():
+    """minfft_batch checks the buffers against the extents its entries
+    address (largest offset plus the entry's items) and refuses a short one
+    before any copy or launch: the output stays as it was."""
+    sizes = [64, 1024, 256]
+    offs = np.concatenate([[0], np.cumsum(sizes)[:-1]])
+    total = int(sum(sizes))
+    b = mf.Batch([("dft", (n,), int(o) * 8, int(o) * 8) for n, o in zip(sizes, offs)], mf.PREC_FP32)
+    x = make_input("dft", (total,), 32, 9)
+    y = np.full(total, 3 + 4j, dtype=np.complex64)
+    with pytest.raises(mf.MinfftError, match="smaller than the entries"):
+        b.run_sync(x[: total - 1], y)
+    with pytest.raises(mf.MinfftError, match="smaller than the entries"):
+        b.run_sync(x, y[: total - 1])
+    assert np.all(y == 3 + 4j)
+    # device buffers: the same check
+    torch = _torch()
+    xd = torch.from_numpy(x).cuda()
+    yd = torch.full((total,), 3 + 4j, dtype=torch.complex64, device="cuda")
+    with pytest.raises(mf.MinfftError, match="smaller than the entries"):
+        b.run_sync(xd, yd[: total - 16])
+    torch.cuda.synchronize()
+    assert bool(torch.all(yd == 3 + 4j))
+    b.run_sync(x, y)  # the exact sizes are accepted
+    b.close()
+    assert not np.any(y == 3 + 4j)
+
+
+def test_async_entry_points_refuse_host_pointers():
+    """minfft_enqueue and minfft_batch_run take device pointers only: host
+    (pageable) arrays are refused with a reason before any launch, and the
+    output array stays as it was."""
+    lib = mf.Minfft(32)
+    aux = mf.Aux(lib, "dft", (256,))
+    x = make_input("dft", (256,), 32, 5)
+    y = np.full(256, 1 + 1j, dtype=np.complex64)
+    with pytest.raises(mf.MinfftError, match="device"):
+        aux.enqueue("dft", x, y)
+    aux.close()
+    assert np.all(y == 1 + 1j)
+    b = mf.Batch([("dft", (256,), 0, 0)], mf.PREC_FP32)
+    with pytest.raises(mf.MinfftError, match="device"):
+        b.run(x.ctypes.data, y.ctypes.data)
+    assert np.all(y == 1 + 1j)
+    # one device and one host pointer: refused as well
+    torch = _torch()
+    xd = torch.from_numpy(x).cuda()
+    with pytest.raises(mf.MinfftError, match="device"):
+        b.run(xd.data_ptr(), y.ctypes.data)
+    torch.cuda.synchronize()
+    assert np.all(y == 1 + 1j)
+    b.close()
