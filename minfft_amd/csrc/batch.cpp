@@ -234,7 +234,8 @@ minfft_batch_plan* minfft_mkaux_batch(int count, const minfft_batch_entry* e, in
         p->out_extent = std::max(p->out_extent, (size_t)x.out_offset + (size_t)oi * ob);
     }
     p->ws = Workspace();
-    if (hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess) return fail("hipStreamCreate failed");
+    // blocking: ordered after the legacy default stream's earlier work (as the plain aux's stream)
+    if (hipStreamCreateWithFlags(&p->stream, hipStreamDefault) != hipSuccess) return fail("hipStreamCreate failed");
     const size_t cs = 2 * real_size(p->prec);
     for (auto& g : p->groups) {
         const unsigned cnt = (unsigned)g.ioff.size();

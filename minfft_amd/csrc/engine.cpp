@@ -911,7 +911,10 @@ Plan* make_plan(Family fam, Prec prec, int d, const int* Ns) {
         free_plan(p);
         return nullptr;
     };
-    if (hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess)
+    // a blocking stream: its work is ordered after work queued earlier on the
+    // legacy default stream, so a synchronous call on device arrays sees what
+    // the caller's default-stream kernels wrote into them
+    if (hipStreamCreateWithFlags(&p->stream, hipStreamDefault) != hipSuccess)
         return fail("hipStreamCreate failed");
     const size_t rs = real_size(prec), cs = 2 * rs;
     // twiddle range and scratch over all axis passes of the family

@@ -412,3 +412,29 @@ def test_back_to_back_synchronous_calls(libs, shape):
         aux("invdft", y, back)
         assert rel_l2(back / n, x) <= tol64(shape), i
     aux.close()
+
+
+def test_aux_stream_ordered_after_default_stream(libs, oracle64):
+    """The aux's own stream (minfft_enqueue with a NULL stream, and the
+    synchronous calls on device arrays) is ordered after work queued on the
+    legacy default stream: an input written there by a long chain of torch
+    kernels just before the call is what the transform reads."""
+    import torch
+    n = 1 << 20
+    for prec in (32, 64):
+        cdt = torch.complex64 if prec == 32 else torch.complex128
+        aux = mf.Aux(libs[prec], "dft", (n,))  # made first: no maker time between the writes and the call
+        print(f"[case] default-stream ordering prec={prec}", flush=True)
+        x = torch.zeros(n, dtype=cdt, device="cuda")
+        for k in range(64):  # ~64 kernels still queued when the transform is enqueued
+            x.add_(1.0 / 64)
+        y = torch.empty_like(x)
+        aux.enqueue("dft", x, y)  # NULL stream: the aux's own
+        z = torch.empty_like(x)
+        aux("dft", x, z)          # synchronous call on device arrays
+        torch.cuda.synchronize()
+        aux.close()
+        # x = 1 everywhere: the spectrum is N at bin 0 and 0 elsewhere
+        for out in (y, z):
+            assert abs(complex(out[0].item()) - n) <= 1e-3 * n
+            assert float(out[1:].abs().max()) <= 1e-2
