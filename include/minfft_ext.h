@@ -59,7 +59,8 @@ typedef struct minfft_batch_plan minfft_batch_plan;
 minfft_batch_plan *minfft_mkaux_batch(int count, const minfft_batch_entry *entries, int prec);
 
 /* Run the batch on device buffers `in` / `out` (device pointers) on the HIP
-   stream `stream` (hipStream_t; NULL = the plan's own stream).  A plan with
+   stream `stream` (hipStream_t; NULL = the plan's own stream, which is
+   ordered after work queued earlier on the legacy default stream).  A plan with
    several units of work (groups launched on their own, persistent launches)
    captures a HIP graph on the first run with a given (in, out, stream) and
    replays it later, its units on concurrent branches; a plan with one unit
@@ -91,8 +92,28 @@ int minfft_batch_schedule(minfft_batch_plan *p, int group, const char *spec);
 int minfft_batch_describe(const minfft_batch_plan *p, char *buf, int len);
 
 /* Enqueue one transform of an aux on device pointers and a caller's stream
-   (hipStream_t; NULL = the aux's own stream) without waiting.  Returns 0 or -1. */
+   (hipStream_t; NULL = the aux's own stream, which is ordered after work
+   queued earlier on the legacy default stream) without waiting.  Returns 0 or -1
+   (also for host pointers, which the kernels cannot read). */
 int minfft_enqueue(int kind, const void *in, void *out, const void *aux, void *stream);
+
+/* Largest scratch minfft_enqueue_noalloc may use: 64 MiB, held by each GPU
+   the library runs on until the process ends. */
+#define MINFFT_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+
+/* minfft_enqueue that allocates nothing.  No counterpart in the reference:
+   it is for callers that cannot allocate device memory during the call
+   (code under HIP graph capture, tight serving loops).  The kernels' scratch
+   (the two-step intermediate of long axes, the complex staging of long
+   real/DCT/DST axes, the multi-dimensional inverse real DFT's temp) comes
+   from a fixed device array of MINFFT_NOALLOC_SCRATCH_BYTES instead of the
+   aux's own buffers; results equal minfft_enqueue's.  The aux's device must
+   be current.  Returns -1 with a reason (minfft_last_error) when the
+   transform needs more scratch than that (e.g. FP32 complex 1D beyond 2^22
+   points, FP64 beyond 2^21), for host pointers and for a bad aux or kind.
+   Every call on one GPU shares that GPU's array: calls that overlap in time
+   on one GPU (other threads, or streams running at once) race on it. */
+int minfft_enqueue_noalloc(int kind, const void *in, void *out, const void *aux, void *stream);
 
 #ifdef __cplusplus
 }

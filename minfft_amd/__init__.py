@@ -28,7 +28,8 @@ MAKERS = [f"minfft_mkaux_{fam}{suf}" for fam in ("dft", "realdft", "t2t3", "t4")
 TRANSFORMS = [f"minfft_{k}" for k in KINDS]
 SYMBOLS = TRANSFORMS + MAKERS + ["minfft_free_aux"]
 EXT_SYMBOLS = ["minfft_last_error", "minfft_mkaux_batch", "minfft_batch_run", "minfft_batch",
-               "minfft_free_batch", "minfft_batch_schedule", "minfft_batch_describe", "minfft_enqueue"]
+               "minfft_free_batch", "minfft_batch_schedule", "minfft_batch_describe", "minfft_enqueue",
+               "minfft_enqueue_noalloc"]
 
 PREC_FP64, PREC_FP32, PREC_SPLIT_FP16 = 0, 1, 2
 
@@ -75,6 +76,9 @@ def engine():
         lib.minfft_enqueue.restype = ctypes.c_int
         lib.minfft_enqueue.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
                                        ctypes.c_void_p, ctypes.c_void_p]
+        lib.minfft_enqueue_noalloc.restype = ctypes.c_int
+        lib.minfft_enqueue_noalloc.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                               ctypes.c_void_p, ctypes.c_void_p]
         _ENGINE = lib
     return _ENGINE
 
@@ -164,6 +168,16 @@ class Aux:
         rc = engine().minfft_enqueue(KIND_ID[kind], _ptr(x), _ptr(y), self.h, stream)
         if rc:
             raise MinfftError(f"minfft_enqueue({kind}): {last_error()}")
+        return y
+
+    def enqueue_noalloc(self, kind, x, y, stream=None):
+        """minfft_enqueue_noalloc (minfft_ext.h): as enqueue, with the
+        kernels' scratch in the library's fixed device array; allocates
+        nothing."""
+        assert FAMILY[kind] == self.family
+        rc = engine().minfft_enqueue_noalloc(KIND_ID[kind], _ptr(x), _ptr(y), self.h, stream)
+        if rc:
+            raise MinfftError(f"minfft_enqueue_noalloc({kind}): {last_error()}")
         return y
 
     def close(self):

@@ -56,6 +56,7 @@ def _units():
     u.append((os.path.join(OBJ, "engine.o"), "engine.cpp", ["-x", "hip"], "engine"))
     u.append((os.path.join(OBJ, "batch.o"), "batch.cpp", ["-x", "hip"], "engine"))
     u.append((os.path.join(OBJ, "autotune.o"), "autotune.cpp", ["-x", "hip"], "engine"))
+    u.append((os.path.join(OBJ, "noalloc.o"), "noalloc.cpp", ["-x", "hip"], "engine"))
     u.append((os.path.join(OBJ, "api64.o"), "api.cpp", ["-x", "hip"], "api64"))
     u.append((os.path.join(OBJ, "api32.o"), "api.cpp", ["-x", "hip", "-DMINFFT_SINGLE=1"], "api32"))
     return u

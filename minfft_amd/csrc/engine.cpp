@@ -1099,18 +1099,51 @@ int run_entries(const Workspace& ws, Prec pr, int kind, int d, const int* Ns, co
     return axis_pass(ws, pr, K_INVREALDFT, logNd, src, 1, out, 1, mk(ax, soff, ooff), st);
 }
 
-int enqueue(Plan& p, int kind, const void* in, void* out, hipStream_t st) {
+// the full-array temp a call of `kind` uses: the multi-dimensional inverse
+// real DFT always, the real DFT when input and output overlap
+static bool uses_tbuf(const Plan& p, int kind, const void* in, const void* out) {
+    if (p.d < 2) return false;
+    if (kind == K_INVREALDFT) return true;
+    return kind == K_REALDFT && overlap(in, in_bytes(p, kind), out, out_bytes(p, kind));
+}
+
+static size_t round256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+size_t scratch_need(const Plan& p, int kind, const void* in, const void* out) {
+    size_t b = p.ws.fs_bytes ? 2 * round256(p.ws.fs_bytes) : 0;
+    if (uses_tbuf(p, kind, in, out)) b += round256(p.tbuf_bytes);
+    return b;
+}
+
+int enqueue_scratch(Plan& p, int kind, const void* in, void* out, hipStream_t st, void* scratch,
+                    size_t scratch_bytes) {
     if (family_of(kind) != p.fam) {
         set_error("aux data does not match the transform kind");
         return -1;
     }
-    void* tbuf = nullptr;
-    if (kind == K_REALDFT && p.d > 1 && overlap(in, in_bytes(p, kind), out, out_bytes(p, kind)))
-        tbuf = p.tbuf;
-    if (kind == K_INVREALDFT && p.d > 1) tbuf = p.tbuf;
+    Workspace ws = p.ws;
+    void* tbuf = uses_tbuf(p, kind, in, out) ? p.tbuf : nullptr;
+    if (scratch) {
+        if (scratch_need(p, kind, in, out) > scratch_bytes) {
+            set_error("scratch too small for this transform");
+            return -1;
+        }
+        char* b = static_cast<char*>(scratch);
+        size_t at = 0;
+        if (ws.fs_bytes) {
+            ws.fbuf = b;
+            ws.gbuf = b + round256(ws.fs_bytes);
+            at = 2 * round256(ws.fs_bytes);
+        }
+        if (tbuf) tbuf = b + at;
+    }
     TuneScope scope(&p.sched);
-    return run_entries(p.ws, p.prec, kind, p.d, p.Ns.data(), in, out, 1, nullptr, nullptr, nullptr, tbuf,
+    return run_entries(ws, p.prec, kind, p.d, p.Ns.data(), in, out, 1, nullptr, nullptr, nullptr, tbuf,
                        nullptr, st);
+}
+
+int enqueue(Plan& p, int kind, const void* in, void* out, hipStream_t st) {
+    return enqueue_scratch(p, kind, in, out, st, nullptr, 0);
 }
 
 // largest host array staged through the aux's pinned bounce buffer

@@ -159,6 +159,14 @@ size_t out_bytes(const Plan& p, int kind);
 
 // run `kind` on device pointers, asynchronously on `st`
 int enqueue(Plan& p, int kind, const void* in, void* out, hipStream_t st);
+// the same with the kernels' scratch (two-step intermediate, long real
+// kinds' complex staging, the multi-dimensional real DFT's temp) carved
+// from `scratch` (device memory of scratch_bytes, 256-byte aligned) instead
+// of the plan's own buffers; -1 if it is too small
+int enqueue_scratch(Plan& p, int kind, const void* in, void* out, hipStream_t st, void* scratch,
+                    size_t scratch_bytes);
+// bytes of scratch such a call needs (0: the transform uses none)
+size_t scratch_need(const Plan& p, int kind, const void* in, const void* out);
 // run `kind` with minfft semantics: any pointer kind, synchronous
 int execute(Plan& p, int kind, const void* in, void* out);
 
