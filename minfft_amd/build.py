@@ -32,7 +32,7 @@ COMMON = [
 ]
 
 HEADERS = ["fft_core.hpp", "kinds.hpp", "tile.hpp", "ktable.hpp", "engine.hpp", "big.hpp", "bigtable.hpp", "mfma.hpp",
-           "autotune.hpp", "split2.hpp", "rsplit.hpp", "multi.hpp"]
+           "autotune.hpp", "split2.hpp", "rsplit.hpp", "multi.hpp", "coop.hpp"]
 
 
 def _newer(target, deps):

@@ -58,7 +58,7 @@ std::string key_of(Prec prec, int kind, int d, const int* n, unsigned count) {
 
 // Cache file: a header line naming the device, its CU count and the
 // library build, then one line per group:
-//   key tpw cfb cfcore wave xcd e32 e4 split split2 fused wpc falt rsplit us_default us_best
+//   key tpw cfb cfcore wave xcd e32 e4 split split2 fused wpc falt rsplit coop coopla us_default us_best
 // A file whose header names another device or build is ignored (and
 // replaced on the next save).  Saves rewrite the whole file into a
 // temporary one and rename it into place, so a reader never sees a partial
@@ -79,7 +79,7 @@ std::string cache_header() {
     std::string build;
     for (const char* c = b; *c; ++c)
         if (*c != ' ') build += *c;
-    std::snprintf(b, sizeof(b), "#minfft-tune v2 arch=%s cus=%d build=%s", arch.c_str(), ncu, build.c_str());
+    std::snprintf(b, sizeof(b), "#minfft-tune v3 arch=%s cus=%d build=%s", arch.c_str(), ncu, build.c_str());
     return b;
 }
 
@@ -98,7 +98,8 @@ void load_cache() {
         r.sched = base_tune();
         Tune& t = r.sched;
         if (ss >> k >> t.tpw >> t.cfb >> t.cfcore >> t.wave >> t.xcd >> t.e32 >> t.e4 >> t.split >> t.split2 >>
-            t.fused >> t.fused_wpc >> t.fused_alt >> t.rsplit >> r.us_default >> r.us_best) {
+            t.fused >> t.fused_wpc >> t.fused_alt >> t.rsplit >> t.coop >> t.coop_la >> r.us_default >>
+            r.us_best) {
             r.tuned = true;
             r.cached = true;
             g_cache[k] = r;
@@ -120,7 +121,8 @@ void save_cache() {
             const Tune& t = r.sched;
             out << kv.first << ' ' << t.tpw << ' ' << t.cfb << ' ' << t.cfcore << ' ' << t.wave << ' ' << t.xcd << ' '
                 << t.e32 << ' ' << t.e4 << ' ' << t.split << ' ' << t.split2 << ' ' << t.fused << ' ' << t.fused_wpc
-                << ' ' << t.fused_alt << ' ' << t.rsplit << ' ' << r.us_default << ' ' << r.us_best << '\n';
+                << ' ' << t.fused_alt << ' ' << t.rsplit << ' ' << t.coop << ' ' << t.coop_la << ' ' << r.us_default
+                << ' ' << r.us_best << '\n';
         }
         if (!out.good()) {
             out.close();
@@ -182,6 +184,8 @@ TuneResult autotune_group(const Workspace& ws, const TuneRequest& r, hipStream_t
         knobs.push_back({&Tune::fused, {1, 0}});
         knobs.push_back({&Tune::fused_alt, {0, 1}});
         knobs.push_back({&Tune::fused_wpc, {2, 1, 4}});
+        knobs.push_back({&Tune::coop, {-1, 1, 0}});
+        knobs.push_back({&Tune::coop_la, {8, 4, 16}});
     }
     if (near_limit) knobs.push_back({&Tune::split2, {-1, 1, 2, 3, 0}});
     if (r.solo && (r.kind == K_DFT || r.kind == K_INVDFT)) {
@@ -310,11 +314,12 @@ TuneResult autotune_group(const Workspace& ws, const TuneRequest& r, hipStream_t
 }
 
 std::string describe_tune(const Tune& t) {
-    char b[192];
+    char b[224];
     std::snprintf(b, sizeof(b),
-                  "tpw=%d cfb=%d cfcore=%d wave=%d xcd=%d e32=%d split=%d split2=%d fused=%d wpc=%d falt=%d rsplit=%d",
+                  "tpw=%d cfb=%d cfcore=%d wave=%d xcd=%d e32=%d split=%d split2=%d fused=%d wpc=%d falt=%d rsplit=%d "
+                  "coop=%d coopla=%d",
                   t.tpw, t.cfb, t.cfcore, t.wave, t.xcd, t.e32, t.split, t.split2, t.fused, t.fused_wpc, t.fused_alt,
-                  t.rsplit);
+                  t.rsplit, t.coop, t.coop_la);
     return b;
 }
 

@@ -290,6 +290,13 @@ minfft_batch_plan* minfft_mkaux_batch(int count, const minfft_batch_entry* e, in
         if (dev_alloc((void**)&p->ws.fbuf, fs_max) != hipSuccess) return fail("hipMalloc batch scratch failed");
         if (dev_alloc((void**)&p->ws.gbuf, fs_max) != hipSuccess) return fail("hipMalloc batch scratch failed");
         p->ws.fs_bytes = fs_max;
+        // counters of the XCD-cooperative kernel: 8 queue heads and one word
+        // per entry of the largest group
+        size_t maxcnt = 0;
+        for (const auto& g : p->groups) maxcnt = std::max(maxcnt, g.ioff.size());
+        const size_t cb = (8 + maxcnt) * sizeof(unsigned);
+        if (dev_alloc((void**)&p->ws.cbuf, cb) != hipSuccess) return fail("hipMalloc batch counters failed");
+        p->ws.cbuf_bytes = cb;
     }
     if (hipStreamSynchronize(p->stream) != hipSuccess) return fail("twiddle generation failed");
     // short 1D complex groups (lengths 2^4 .. 2^12): one persistent launch

@@ -5,6 +5,7 @@
 // (split2.hpp), the split-FP16 matrix-core 512-point tile (mfma.hpp) and the
 // persistent short-length batch kernel (multi.hpp).
 #include "big.hpp"
+#include "coop.hpp"
 #include "mfma.hpp"
 #include "split2.hpp"
 #include "multi.hpp"
@@ -28,6 +29,21 @@ static void reg_one(BigTable& t, int prec, int variant = 0) {
     t.lds[prec][lg] = big_lds_bytes<T, L1, L2, BT>();
     t.bt[prec][lg] = BT;
     t.wpc[prec][lg] = (4 * WPE * 64) / BT;  // work-groups per CU the registers allow
+}
+
+__global__ void coop_reset_kernel(unsigned* ctr, unsigned n) {
+    const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) ctr[i] = 0u;
+}
+
+template <typename T, int L1, int L2, int BT, int WPE>
+static void reg_coop(BigTable& t, int prec) {
+    const int lg = L1 + L2;
+    t.coop[prec][lg][0] = reinterpret_cast<const void*>(&coop_kernel<T, L1, L2, false, BT, WPE>);
+    t.coop[prec][lg][1] = reinterpret_cast<const void*>(&coop_kernel<T, L1, L2, true, BT, WPE>);
+    t.coop_lds[prec][lg] = coop_lds_bytes<T, L1, L2, BT>();
+    t.coop_bt[prec][lg] = BT;
+    t.coop_wpc[prec][lg] = (4 * WPE * 64) / BT;
 }
 
 // variant 0: 16 points per lane; variant 1 (E = 32): 32 points per lane
@@ -75,6 +91,13 @@ void register_big(BigTable& t) {
     reg_one<double, 7, 7, 256, 2>(t, 0);
     reg_one<double, 8, 7, 256, 2>(t, 0);
     reg_one<double, 8, 8, 256, 2>(t, 0);
+    // XCD-cooperative forms of the same factorisations
+    t.coop_reset = reinterpret_cast<const void*>(&coop_reset_kernel);
+    reg_coop<float, 8, 7, 512, 4>(t, 1);
+    reg_coop<float, 8, 8, 512, 4>(t, 1);
+    reg_coop<double, 7, 7, 256, 2>(t, 0);
+    reg_coop<double, 8, 7, 256, 2>(t, 0);
+    reg_coop<double, 8, 8, 256, 2>(t, 0);
     // smaller work-groups (4 per CU) for launch-shape comparison
     reg_one<float, 8, 7, 256, 4>(t, 1, 1);
     reg_one<float, 8, 8, 256, 4>(t, 1, 1);

@@ -32,6 +32,13 @@ struct BigTable {
     const void* rsplit[2][17][4][2][2];  // [prec][log2 N][log2 P - 1][B - 1][inverse]
     int rsplit_bt[2][17][4][2];
     size_t rsplit_lds[2][17][4][2];
+    // XCD-cooperative two-step kernels (coop.hpp), by log2 N, and the reset
+    // kernel of their counters
+    const void* coop[2][17][2];  // [prec][log2 N][inverse]
+    size_t coop_lds[2][17];
+    int coop_bt[2][17];
+    int coop_wpc[2][17];
+    const void* coop_reset;
     // persistent launch for the short 1D complex groups of a batch (multi.hpp)
     const void* multi[2][2];  // [prec][inverse]
     size_t multi_lds[2];

@@ -29,6 +29,7 @@
 #include "ktable.hpp"
 #include "tile.hpp"
 #include "big.hpp"
+#include "coop.hpp"
 #include "multi.hpp"
 #include "autotune.hpp"
 
@@ -181,7 +182,7 @@ bool is_device_pointer(const void* p) {
 // ------------------------------------------------------------ tile tuning
 // Process-wide schedule: defaults, then MINFFT_TUNE="key=value,..." (keys:
 // tpw, cfb, cfcore, wave, xcd, e32, e4, stage, chunk, split, split2, fused,
-// wpc, falt, rsplit -- the fields of Tune in engine.hpp).
+// wpc, falt, rsplit, coop, coopla -- the fields of Tune in engine.hpp).
 static Tune g_tune;
 static std::once_flag g_tune_once;
 static thread_local const Tune* t_tune = nullptr;
@@ -218,6 +219,8 @@ bool parse_tune(const char* spec, Tune& t) {
         else if (k == "wpc" && v > 0) t.fused_wpc = v;
         else if (k == "falt") t.fused_alt = v;
         else if (k == "rsplit") t.rsplit = v;
+        else if (k == "coop") t.coop = v;
+        else if (k == "coopla" && v >= 1 && v <= 64) t.coop_la = v;
         else ok = false;
     }
     return ok;
@@ -574,6 +577,17 @@ static int launch_rsplit(const Workspace& ws, Prec prec, int kind, int logL, con
     return launch_fn(fn, (unsigned)grid, (unsigned)bt.rsplit_bt[pi][logL][lp - 1][b - 1], args, lds, st);
 }
 
+// Queues of the XCD-cooperative kernel: one per XCD, which must each have
+// an L2 of their own -- 8 on a whole MI355X (256 CUs, 32 per XCD), 1 on a
+// one-XCD partition; 0 (kernel not used) for any other layout.
+unsigned coop_queues() {
+    if (!ensure_device()) return 0;
+    const int ncu = device().ncu;
+    if (ncu == 256) return 8;
+    if (ncu == 32) return 1;
+    return 0;
+}
+
 int multi_grid(Prec prec) {
     if (!ensure_device()) return 0;
     const BigTable& bt = bigtable();
@@ -696,6 +710,44 @@ static int c2c_axis(const Workspace& ws, Prec prec, int kind, int logL, const vo
     if ((lv.ioff || lv.ooff) && fs_pos == 0) {
         set_error("internal: offset table level clash");
         return -1;
+    }
+    // Batches of contiguous 1D entries with a slot per entry in the scratch:
+    // the XCD-cooperative two-step kernel (coop.hpp), intermediates in L2.
+    {
+        const BigTable& bt = bigtable();
+        const Tune& tn = tune();
+        const int pi = prec == PREC_F64 ? 0 : 1;
+        const unsigned cnt = lv.n ? lv.B[0] : 1;
+        const unsigned nq = coop_queues();
+        const void* fn = logL <= 16 ? bt.coop[pi][logL][kind == K_INVDFT] : nullptr;
+        const size_t cs = prec == PREC_F64 ? 16 : 8;
+        const bool coop_on = tn.coop < 0 ? prec == PREC_F64 : tn.coop != 0;
+        if (fn && coop_on && nq && lv.n == 1 && lv.ioff && lv.ooff && lv.foff && e == 1 && ws.cbuf &&
+            cnt >= (unsigned)device().ncu && cnt < (1u << 24) &&
+            (size_t)(nq + cnt) * sizeof(unsigned) <= ws.cbuf_bytes &&
+            (size_t)cnt * ((size_t)1 << logL) * cs <= ws.fs_bytes &&
+            bt.coop_lds[pi][logL] <= (size_t)device().lds_max) {
+            unsigned* ctr = ws.cbuf;
+            unsigned nctr = nq + cnt;
+            void* rargs[] = {&ctr, &nctr};
+            if (launch_fn(bt.coop_reset, (nctr + 255) / 256, 256, rargs, 0, st)) return -1;
+            CoopArgs C;
+            C.in = in;
+            C.out = out;
+            C.scratch = ws.fbuf;
+            C.tw = ws.tw;
+            C.tw64 = (const double2*)ws.tw64;
+            C.in_off = lv.ioff;
+            C.out_off = lv.ooff;
+            C.slot_off = lv.foff;
+            C.ctr = ctr;
+            C.count = cnt;
+            C.nq = nq;
+            C.lookahead = (unsigned)tn.coop_la;
+            const unsigned grid = (unsigned)(device().ncu * std::max(1, bt.coop_wpc[pi][logL]));
+            void* args[] = {&C};
+            return launch_fn(fn, grid, (unsigned)bt.coop_bt[pi][logL], args, bt.coop_lds[pi][logL], st);
+        }
     }
     // Batches of contiguous 1D entries: one fused launch, a work-group per
     // transform, intermediate in a per-work-group scratch slot.
@@ -846,6 +898,7 @@ void free_workspace(Workspace& ws) {
     if (ws.tw64) (void)hipFree(ws.tw64);
     if (ws.fbuf) (void)hipFree(ws.fbuf);
     if (ws.gbuf) (void)hipFree(ws.gbuf);
+    if (ws.cbuf) (void)hipFree(ws.cbuf);
     ws = Workspace();
 }
 

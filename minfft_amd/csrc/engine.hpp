@@ -65,6 +65,8 @@ struct Workspace {
     void* fbuf = nullptr;      // two-step intermediate
     void* gbuf = nullptr;      // two-step complex staging for real kinds
     size_t fs_bytes = 0;
+    unsigned* cbuf = nullptr;  // counters of the XCD-cooperative kernel (batch plans)
+    size_t cbuf_bytes = 0;
     int split16 = 0;           // FP32 data, radix-16 stages as split-FP16 MFMA products
 };
 
@@ -94,6 +96,9 @@ struct Tune {
     int rsplit = 0;     // residue-split single-pass kernel for contiguous axes:
                         // 10*log2 P + residues per work-group (e.g. 31 = 8 blocks,
                         // one residue), where instantiated; 0 = off
+    int coop = -1;      // XCD-cooperative form of the fused two-step kernel (coop.hpp):
+                        // 1 on, 0 off, -1 by precision (FP64 on, FP32 off: the measured best)
+    int coop_la = 8;    // its lookahead: rounds between a transform's two steps
 };
 struct Plan {
     unsigned magic = 0x6d667870u;
@@ -121,8 +126,8 @@ struct Plan {
 // the schedule in force on this thread
 const Tune& tune();
 // applies "key=value,..." (the Tune fields: tpw, cfb, wave, xcd, cfcore,
-// e32, stage, chunk, split, split2, fused, wpc, falt, rsplit) to t; false on an
-// unknown key
+// e32, stage, chunk, split, split2, fused, wpc, falt, rsplit, coop, coopla) to
+// t; false on an unknown key
 bool parse_tune(const char* spec, Tune& t);
 // the process-wide schedule (defaults + MINFFT_TUNE)
 const Tune& base_tune();
@@ -183,6 +188,9 @@ int axis_pass(const Workspace& ws, Prec prec, int kind, int logN,
 // (counter reset, then the kernel) on st
 struct MultiArgs;
 int multi_grid(Prec prec);
+// queues (XCDs with an L2 of their own) of the XCD-cooperative two-step
+// kernel on this device; 0 = the kernel is not used here
+unsigned coop_queues();
 int launch_multi(Prec prec, bool inv, const MultiArgs& a, unsigned grid, hipStream_t st);
 // schedule codes (Tune::rsplit) of the residue-split kernels instantiated
 // for complex transforms of length 2^logN whose LDS fits this device

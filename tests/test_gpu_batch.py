@@ -562,3 +562,50 @@ def test_async_entry_points_refuse_host_pointers():
     torch.cuda.synchronize()
     assert np.all(y == 1 + 1j)
     b.close()
+
+
+@pytest.mark.parametrize("prec,logn", [(32, 16), (32, 15), (64, 16), (64, 15), (64, 14)])
+def test_cooperative_kernel_matches_fused(oracle64, prec, logn):
+    """The XCD-cooperative two-step kernel (coop=1; the default for FP64) against the
+    fused one-work-group-per-transform kernel (coop=0): the same arithmetic,
+    so every entry's output is identical bit for bit, with the lookahead at
+    1 (most step-2 chunks deferred to the work-group finishing step 1), 5 and
+    64 (none deferred), out of place and in place; a spread of entries also
+    against the oracle.  FP32 2^15 and FP64 2^14 run with the radix-2-split
+    kernel off so that the two-step kernels take the group."""
+    torch = _torch()
+    n = 1 << logn
+    cnt = 300
+    cdt = torch.complex64 if prec == 32 else torch.complex128
+    g = torch.Generator(device="cuda")
+    g.manual_seed(logn)
+    x = torch.complex(torch.rand(cnt * n, generator=g, device="cuda", dtype=torch.float64) * 2 - 1,
+                      torch.rand(cnt * n, generator=g, device="cuda", dtype=torch.float64) * 2 - 1).to(cdt)
+    esz = 8 if prec == 32 else 16
+    kinds = ["dft" if i % 3 else "invdft" for i in range(cnt)]
+    b = mf.Batch([(kinds[i], (n,), i * n * esz, i * n * esz) for i in range(cnt)],
+                 mf.PREC_FP32 if prec == 32 else mf.PREC_FP64)
+    extra = ",split2=0" if (prec, logn) in ((32, 15), (64, 14)) else ""
+    groups = b.describe().count("\n")
+    outs = {}
+    for spec in ("coop=0", "coop=1,coopla=1", "coop=1,coopla=8", "coop=1,coopla=64"):
+        for gi in range(groups):
+            b.schedule(gi, spec + extra)
+        print(f"[case] coop prec={prec} N=2^{logn} {spec}", flush=True)
+        y = torch.full_like(x, float("nan"))
+        b.run(x.data_ptr(), y.data_ptr())
+        b.run(x.data_ptr(), y.data_ptr())  # replayed
+        z = x.clone()
+        b.run(z.data_ptr(), z.data_ptr())  # in place
+        torch.cuda.synchronize()
+        assert torch.equal(y, z), spec
+        outs[spec] = y
+    for spec, y in outs.items():
+        assert torch.equal(y, outs["coop=0"]), spec
+    b.close()
+    yh = outs["coop=1,coopla=8"].cpu().numpy().reshape(cnt, n)
+    xh = x.cpu().numpy().reshape(cnt, n)
+    bar = TOL32 if prec == 32 else tol64((n,))
+    for i in sorted({0, 1, 7, 8, 9, 150, cnt - 2, cnt - 1}):
+        ref = oracle64.run(kinds[i], xh[i].astype(np.complex128), (n,))
+        assert rel_l2(yh[i], ref) <= bar, i

@@ -18,7 +18,8 @@ SPECS = ["", "tpw=128", "tpw=512", "cfb=64", "cfb=256", "xcd=0", "wave=0",
          "fused=0", "falt=1", "wpc=1", "wpc=4", "split=-1", "split=1",
          "split2=0", "split2=0,split=-1", "split2=0,split=1", "split2=1", "split2=2", "split2=3",
          "fused=0,split=-1", "fused=0,split=1",
-         "rsplit=11", "rsplit=12", "rsplit=21", "rsplit=22", "rsplit=31", "rsplit=32", "rsplit=41", "e4=0"]
+         "rsplit=11", "rsplit=12", "rsplit=21", "rsplit=22", "rsplit=31", "rsplit=32", "rsplit=41", "e4=0",
+         "coop=0", "coop=1", "coop=1,coopla=1", "coop=1,coopla=4", "coop=1,coopla=16", "coop=1,coopla=64"]
 
 
 class Packed:
