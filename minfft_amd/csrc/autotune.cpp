@@ -58,7 +58,7 @@ std::string key_of(Prec prec, int kind, int d, const int* n, unsigned count) {
 
 // Cache file: a header line naming the device, its CU count and the
 // library build, then one line per group:
-//   key tpw cfb cfcore wave xcd e32 e4 split split2 fused wpc falt rsplit coop coopla us_default us_best
+//   key tpw cfb cfcore wave xcd e32 e4 split split2 fused wpc falt rsplit coop coopla mall us_default us_best
 // A file whose header names another device or build is ignored (and
 // replaced on the next save).  Saves rewrite the whole file into a
 // temporary one and rename it into place, so a reader never sees a partial
@@ -79,7 +79,7 @@ std::string cache_header() {
     std::string build;
     for (const char* c = b; *c; ++c)
         if (*c != ' ') build += *c;
-    std::snprintf(b, sizeof(b), "#minfft-tune v3 arch=%s cus=%d build=%s", arch.c_str(), ncu, build.c_str());
+    std::snprintf(b, sizeof(b), "#minfft-tune v4 arch=%s cus=%d build=%s", arch.c_str(), ncu, build.c_str());
     return b;
 }
 
@@ -98,8 +98,8 @@ void load_cache() {
         r.sched = base_tune();
         Tune& t = r.sched;
         if (ss >> k >> t.tpw >> t.cfb >> t.cfcore >> t.wave >> t.xcd >> t.e32 >> t.e4 >> t.split >> t.split2 >>
-            t.fused >> t.fused_wpc >> t.fused_alt >> t.rsplit >> t.coop >> t.coop_la >> r.us_default >>
-            r.us_best) {
+            t.fused >> t.fused_wpc >> t.fused_alt >> t.rsplit >> t.coop >> t.coop_la >> t.mall_mb >>
+            r.us_default >> r.us_best) {
             r.tuned = true;
             r.cached = true;
             g_cache[k] = r;
@@ -121,7 +121,8 @@ void save_cache() {
             const Tune& t = r.sched;
             out << kv.first << ' ' << t.tpw << ' ' << t.cfb << ' ' << t.cfcore << ' ' << t.wave << ' ' << t.xcd << ' '
                 << t.e32 << ' ' << t.e4 << ' ' << t.split << ' ' << t.split2 << ' ' << t.fused << ' ' << t.fused_wpc
-                << ' ' << t.fused_alt << ' ' << t.rsplit << ' ' << t.coop << ' ' << t.coop_la << ' ' << r.us_default
+                << ' ' << t.fused_alt << ' ' << t.rsplit << ' ' << t.coop << ' ' << t.coop_la << ' ' << t.mall_mb
+                << ' ' << r.us_default
                 << ' ' << r.us_best << '\n';
         }
         if (!out.good()) {
@@ -255,6 +256,9 @@ TuneResult autotune_group(const Workspace& ws, const TuneRequest& r, hipStream_t
     auto measure = [&](const Tune& t) -> float {
         TuneScope scope(&t);
         auto once = [&]() {
+            if (cs == 1)  // a single transform, as the batch runs a one-entry group
+                return run_entries(ws, r.prec, r.kind, r.d, r.n, din, dout, 1, nullptr, nullptr, nullptr, r.tbuf,
+                                   nullptr, st);
             return run_entries(ws, r.prec, r.kind, r.d, r.n, din, dout, cs, dio, doo, r.d_foff, r.tbuf, r.d_toff, st);
         };
         if (once()) {
@@ -317,9 +321,9 @@ std::string describe_tune(const Tune& t) {
     char b[224];
     std::snprintf(b, sizeof(b),
                   "tpw=%d cfb=%d cfcore=%d wave=%d xcd=%d e32=%d split=%d split2=%d fused=%d wpc=%d falt=%d rsplit=%d "
-                  "coop=%d coopla=%d",
+                  "coop=%d coopla=%d mall=%d",
                   t.tpw, t.cfb, t.cfcore, t.wave, t.xcd, t.e32, t.split, t.split2, t.fused, t.fused_wpc, t.fused_alt,
-                  t.rsplit, t.coop, t.coop_la);
+                  t.rsplit, t.coop, t.coop_la, t.mall_mb);
     return b;
 }
 

@@ -390,6 +390,13 @@ static int enqueue_batch(minfft_batch_plan* p, const void* in, void* out, hipStr
         }
         BatchGroup& g = p->groups[u];
         TuneScope scope(&g.tune.sched);
+        if (g.ioff.size() == 1) {
+            // one entry: a single transform at known addresses (no tables)
+            const void* ip = static_cast<const char*>(in) + g.ioff[0] * (long long)item_in_bytes(g.kind, p->prec);
+            void* op = static_cast<char*>(out) + g.ooff[0] * (long long)item_out_bytes(g.kind, p->prec);
+            return run_entries(p->ws, p->prec, g.kind, g.d, g.n, ip, op, 1, nullptr, nullptr, nullptr, g.tbuf,
+                               nullptr, s);
+        }
         return run_entries(p->ws, p->prec, g.kind, g.d, g.n, in, out, (unsigned)g.ioff.size(), g.d_ioff, g.d_ooff,
                            g.d_foff, g.tbuf, g.d_toff, s);
     };

@@ -99,6 +99,9 @@ struct Tune {
     int coop = -1;      // XCD-cooperative form of the fused two-step kernel (coop.hpp):
                         // 1 on, 0 off, -1 by precision (FP64 on, FP32 off: the measured best)
     int coop_la = 8;    // its lookahead: rounds between a transform's two steps
+    int mall_mb = 0;    // multi-dimensional passes on chunks of entries / slabs of
+                        // about this many MiB, so the later passes read the MALL (0 = off:
+                        // measured slower at every size, see run_entries)
 };
 struct Plan {
     unsigned magic = 0x6d667870u;
@@ -126,8 +129,8 @@ struct Plan {
 // the schedule in force on this thread
 const Tune& tune();
 // applies "key=value,..." (the Tune fields: tpw, cfb, wave, xcd, cfcore,
-// e32, stage, chunk, split, split2, fused, wpc, falt, rsplit, coop, coopla) to
-// t; false on an unknown key
+// e32, stage, chunk, split, split2, fused, wpc, falt, rsplit, coop, coopla,
+// mall) to t; false on an unknown key
 bool parse_tune(const char* spec, Tune& t);
 // the process-wide schedule (defaults + MINFFT_TUNE)
 const Tune& base_tune();
