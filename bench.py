@@ -370,6 +370,20 @@ def config_c2(a, torch, mf):
            "dct2": float(np.linalg.norm(cd.cpu().numpy() - cr) / np.linalg.norm(cr)),
            "invrealdft_round_trip": float(np.linalg.norm(yd.cpu().numpy() / N - x) / np.linalg.norm(x)),
            "dct3_of_dct2": float(np.linalg.norm(dd.cpu().numpy() / (4 * N) - x) / np.linalg.norm(x))}
+    # host arrays through the synchronous API (PCIe-inclusive): the aux's
+    # pinned bounce buffer in 4 MiB pieces, host copies overlapping the DMA,
+    # against one piece (MINFFT_HOST_PIPE=0)
+    host = {}
+    zh = np.empty((n, n // 2 + 1), dtype=np.complex128)
+    for pipe in ("1", "0"):
+        os.environ["MINFFT_HOST_PIPE"] = pipe
+        ar("realdft", x, zh)
+        t0 = time.perf_counter()
+        for _ in range(10):
+            ar("realdft", x, zh)
+        host["pipelined" if pipe == "1" else "one_piece"] = round((time.perf_counter() - t0) / 10 * 1e3, 3)
+    os.environ.pop("MINFFT_HOST_PIPE", None)
+    host["unit"] = "ms per realdft of 2048x2048 FP64 host arrays (copies included)"
     cpu = None
     if ref_available(64, fast=True):
         R = Reference(64, fast=True)
@@ -390,6 +404,7 @@ def config_c2(a, torch, mf):
     return _line("2D real DFT 2048x2048 FP64 GFLOP/s", round(flops / (ms * 1e-3) / 1e9, 3), "GFLOP/s", ms,
                  a.steps, a.warmup, "f64", "C2: 2048x2048 realdft/invrealdft + DCT-2/DCT-3, FP64",
                  {"per_transform": per, "rel_l2": err, "tolerance": tol, "oracle_seconds": round(oracle_s, 2),
+                  "host_arrays": host,
                   "roofline": {"bound": "hbm", "achieved": per["realdft"]["alg_hbm_gbs"], "peak": HBM_PEAK_GBS,
                                "unit": "GB/s", "frac": round(per["realdft"]["alg_hbm_gbs"] / HBM_PEAK_GBS, 4),
                                "traffic": None, "kernel": "realdft (2 axis passes)"},

@@ -307,6 +307,16 @@ static void fill_levels(TileArgs& A, const Levels& lv, int* fs_level = nullptr) 
     A.lgC = 0;
 }
 
+// A request that the next one-work-group tile launch of this thread store a
+// completion word for the host (signal_host in tile.hpp); `used` tells
+// whether a launch took it.
+struct HostSignal {
+    unsigned* flag = nullptr;
+    unsigned value = 0;
+    bool used = false;
+};
+static thread_local HostSignal* t_signal = nullptr;
+
 // Passes over at least this many bytes use non-temporal I/O where the
 // kernel has the form (the MALL of MI355X holds 256 MiB)
 constexpr unsigned long long kStreamBytes = 256ull << 20;
@@ -468,6 +478,15 @@ static int launch_tile(Prec prec, int kind, int logL, const void* tw, const void
     // wavefront-local core exchanges need a row-mapped core; staged column
     // I/O keeps its work-group barriers (tile.hpp)
     A.wave = (tn.wave && !A.cfast) ? 1 : 0;
+    const BigTable& btab = bigtable();
+    const bool plain_tile = fn != btab.mfma512[0] && fn != btab.mfma512[1] && fn != btab.mfma512cf[0] &&
+                            fn != btab.mfma512cf[1] && fn != btab.mfma512nt[0] && fn != btab.mfma512nt[1] &&
+                            fn != btab.mfma512cfnt[0] && fn != btab.mfma512cfnt[1];
+    if (t_signal && !t_signal->used && tiles == 1 && plain_tile) {
+        A.hflag = t_signal->flag;
+        A.hval = t_signal->value;
+        t_signal->used = true;
+    }
     void* args[] = {&A};
     return launch_fn(fn, (unsigned)tiles, (unsigned)(C * TPT), args, (size_t)C * sc * cs, st);
 }
@@ -872,6 +891,7 @@ int axis_pass(const Workspace& ws, Prec prec, int kind, int logN, const void* in
     fill_levels(B, lv);
     B.ntrans = (unsigned)ntr;
     B.tw = ws.tw;
+    B.tw64 = (const double2*)ws.tw64;  // the post-twiddle is computed in FP64
     B.in = ws.gbuf;
     B.in_off = nullptr;
     B.out = out;
@@ -928,6 +948,7 @@ void free_plan(Plan* p) {
     if (p->stage) (void)hipFree(p->stage);
     if (p->hstage) (void)hipHostFree(p->hstage);
     if (p->hflag) (void)hipHostFree(p->hflag);
+    for (auto e : p->pev) (void)hipEventDestroy(e);
     p->magic = 0;
     delete p;
 }
@@ -1263,6 +1284,13 @@ int enqueue(Plan& p, int kind, const void* in, void* out, hipStream_t st) {
 
 // largest host array staged through the aux's pinned bounce buffer
 constexpr size_t kPinnedStageMax = (size_t)64 << 20;
+// pieces of the pipelined bounce copies (at most kMaxPieces of them)
+constexpr size_t kPipePiece = (size_t)4 << 20;
+constexpr size_t kMaxPieces = kPinnedStageMax / kPipePiece;
+static bool host_pipe_enabled() {
+    const char* e = std::getenv("MINFFT_HOST_PIPE");
+    return !(e && std::atoi(e) == 0);
+}
 // largest transform (points in all) whose kernels work on the pinned buffer
 // directly (one tile pass per axis for every kind at either precision)
 constexpr long long kZeroCopyMaxPoints = 4096;
@@ -1280,32 +1308,96 @@ static bool host_poll_enabled() {
     return !(e && std::atoi(e) == 0);
 }
 
-static int wait_kernels(Plan& p) {
-    if (host_poll_enabled()) {
-        if (!p.hflag) {
-            if (hipHostMalloc((void**)&p.hflag, 64, hipHostMallocCoherent) != hipSuccess) {
-                (void)hipGetLastError();
-                p.hflag = nullptr;
-            } else {
-                __atomic_store_n(p.hflag, 0u, __ATOMIC_RELAXED);
-                p.hseq = 0;
-            }
-        }
-        if (p.hflag) {
-            if (++p.hseq == 0u) ++p.hseq;  // 0 is the word's initial value
-            const unsigned v = p.hseq;
-            if (launch_host_flag(p.hflag, v, p.stream) == hipSuccess) {
-                const auto t0 = std::chrono::steady_clock::now();
-                for (;;) {
-                    if (__atomic_load_n(p.hflag, __ATOMIC_ACQUIRE) == v) return 0;
-                    if (std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(2)) break;
+static int wait_kernels(Plan& p);
+
+// MINFFT_SIGNAL=0: the completion word always comes from a second launch
+static bool host_signal_enabled() {
+    const char* e = std::getenv("MINFFT_SIGNAL");
+    return !(e && std::atoi(e) == 0);
+}
+
+// polls the plan's completion word for `v` (bounded: false after 2 ms)
+static bool poll_flag(const Plan& p, unsigned v) {
+    const auto t0 = std::chrono::steady_clock::now();
+    for (;;) {
+        if (__atomic_load_n(p.hflag, __ATOMIC_ACQUIRE) == v) return true;
+        if (std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(2)) return false;
 #if defined(__x86_64__)
-                    __builtin_ia32_pause();
+        __builtin_ia32_pause();
 #endif
-                }
-            } else {
-                (void)hipGetLastError();
-            }
+    }
+}
+
+// next value for the completion word (0 is its initial value); the word is
+// allocated on first use (false: not available)
+static bool next_flag_value(Plan& p, unsigned& v) {
+    if (!p.hflag) {
+        if (hipHostMalloc((void**)&p.hflag, 64, hipHostMallocCoherent) != hipSuccess) {
+            (void)hipGetLastError();
+            p.hflag = nullptr;
+            return false;
+        }
+        __atomic_store_n(p.hflag, 0u, __ATOMIC_RELAXED);
+        p.hseq = 0;
+    }
+    if (++p.hseq == 0u) ++p.hseq;
+    v = p.hseq;
+    return true;
+}
+
+// true when `kind` on this plan is exactly one tile launch (1D, at most one
+// tile long, neither the radix-2-split nor the residue-split kernel), so that
+// the launch itself can store the completion word
+static bool single_tile_launch(const Plan& p, int kind) {
+    if (p.d != 1 || !ensure_device()) return false;
+    const int logN = ilog2i(p.Ns[0]);
+    const int maxlog = device().maxlog[p.prec];
+    if (kind == K_DFT || kind == K_INVDFT) {
+        if (logN > maxlog || p.sched.rsplit > 0) return false;
+        const int sp2 = p.sched.split2 >= 0 ? p.sched.split2 : (p.prec == PREC_F64 ? 2 : 3);
+        if ((sp2 >= 2 && logN == maxlog) || (sp2 >= 3 && logN == maxlog - 1)) return false;
+        return true;
+    }
+    return logN >= 1 && logN - 1 <= maxlog;
+}
+
+// Enqueue `kind` and wait for it; a single tile launch stores the
+// completion word itself (no second launch), otherwise wait_kernels
+static int run_and_wait(Plan& p, int kind, const void* in, void* out) {
+    unsigned v = 0;
+    if (host_poll_enabled() && host_signal_enabled() && single_tile_launch(p, kind) && next_flag_value(p, v)) {
+        HostSignal sig;
+        sig.flag = p.hflag;
+        sig.value = v;
+        t_signal = &sig;
+        const int rc = enqueue(p, kind, in, out, p.stream);
+        t_signal = nullptr;
+        if (rc) {
+            (void)hipStreamSynchronize(p.stream);
+            return rc;
+        }
+        if (sig.used) {
+            if (poll_flag(p, v)) return 0;
+            HIPCHK(hipStreamSynchronize(p.stream), "transform");
+            return 0;
+        }
+        return wait_kernels(p);
+    }
+    const int rc = enqueue(p, kind, in, out, p.stream);
+    if (rc) {
+        (void)hipStreamSynchronize(p.stream);
+        return rc;
+    }
+    return wait_kernels(p);
+}
+
+static int wait_kernels(Plan& p) {
+    unsigned v = 0;
+    if (host_poll_enabled() && next_flag_value(p, v)) {
+        if (launch_host_flag(p.hflag, v, p.stream) == hipSuccess) {
+            if (poll_flag(p, v)) return 0;
+        } else {
+            (void)hipGetLastError();
         }
     }
     HIPCHK(hipStreamSynchronize(p.stream), "transform");
@@ -1322,14 +1414,7 @@ int execute(Plan& p, int kind, const void* in, void* out) {
     if (hipGetDevice(&cur) != hipSuccess || cur != p.dev) (void)hipSetDevice(p.dev);
     const size_t ib = in_bytes(p, kind), ob = out_bytes(p, kind);
     const bool in_host = !is_device_pointer(in), out_host = !is_device_pointer(out);
-    if (!in_host && !out_host) {
-        int rc = enqueue(p, kind, in, out, p.stream);
-        if (rc) {
-            (void)hipStreamSynchronize(p.stream);
-            return rc;
-        }
-        return wait_kernels(p);
-    }
+    if (!in_host && !out_host) return run_and_wait(p, kind, in, out);
     const size_t need = std::max(ib, ob);
     if (p.stage_bytes < need) {
         if (p.stage) (void)hipFree(p.stage);
@@ -1362,18 +1447,22 @@ int execute(Plan& p, int kind, const void* in, void* out) {
         // axis) run their kernels on the pinned buffer itself, in place (the
         // passes are in-place safe, as for device arrays), with no DMA copy
         std::memcpy(hb, in, ib);
-        int rc = enqueue(p, kind, hb, hb, p.stream);
-        if (rc) {
-            (void)hipStreamSynchronize(p.stream);
-            return rc;
-        }
-        if (wait_kernels(p)) return -1;
+        if (run_and_wait(p, kind, hb, hb)) return -1;
         std::memcpy(out, hb, ob);
         return 0;
     }
+    // Through the bounce buffer in pieces: the host copy of one piece
+    // overlaps the DMA of the piece before it, in both directions
+    // (MINFFT_HOST_PIPE=0: one piece)
+    const size_t piece = host_pipe_enabled() ? kPipePiece : (size_t)-1;
     if (in_host && hb) {
-        std::memcpy(hb, in, ib);
-        HIPCHK(hipMemcpyAsync(p.stage, hb, ib, hipMemcpyHostToDevice, p.stream), "copy in");
+        for (size_t off = 0; off < ib; off += piece) {
+            const size_t len = std::min(piece, ib - off);
+            std::memcpy(static_cast<char*>(hb) + off, static_cast<const char*>(in) + off, len);
+            HIPCHK(hipMemcpyAsync(static_cast<char*>(p.stage) + off, static_cast<char*>(hb) + off, len,
+                                  hipMemcpyHostToDevice, p.stream),
+                   "copy in");
+        }
     } else {
         HIPCHK(hipMemcpyAsync(p.stage, in, ib, hipMemcpyDefault, p.stream), "copy in");
     }
@@ -1383,9 +1472,26 @@ int execute(Plan& p, int kind, const void* in, void* out) {
         return rc;
     }
     if (out_host && hb) {
-        HIPCHK(hipMemcpyAsync(hb, p.stage, ob, hipMemcpyDeviceToHost, p.stream), "copy out");
-        HIPCHK(hipStreamSynchronize(p.stream), "transform");
-        std::memcpy(out, hb, ob);
+        // (ob <= kPinnedStageMax here, so at most kMaxPieces pieces)
+        const size_t np = piece >= ob ? 1 : std::min(kMaxPieces, (ob + piece - 1) / piece);
+        const size_t pl = np > 1 ? (ob + np - 1) / np : std::max<size_t>(ob, 1);
+        while (p.pev.size() < np) {
+            hipEvent_t e;
+            HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
+            p.pev.push_back(e);
+        }
+        for (size_t k = 0, off = 0; off < ob; ++k, off += pl) {
+            const size_t len = std::min(pl, ob - off);
+            HIPCHK(hipMemcpyAsync(static_cast<char*>(hb) + off, static_cast<char*>(p.stage) + off, len,
+                                  hipMemcpyDeviceToHost, p.stream),
+                   "copy out");
+            HIPCHK(hipEventRecord(p.pev[k], p.stream), "hipEventRecord");
+        }
+        for (size_t k = 0, off = 0; off < ob; ++k, off += pl) {
+            const size_t len = std::min(pl, ob - off);
+            HIPCHK(hipEventSynchronize(p.pev[k]), "transform");
+            std::memcpy(static_cast<char*>(out) + off, static_cast<const char*>(hb) + off, len);
+        }
         return 0;
     }
     HIPCHK(hipMemcpyAsync(out, p.stage, ob, hipMemcpyDefault, p.stream), "copy out");

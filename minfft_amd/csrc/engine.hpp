@@ -120,6 +120,7 @@ struct Plan {
     size_t hstage_bytes = 0;  // up to kPinnedStageMax bytes)
     unsigned* hflag = nullptr;  // completion word in coherent pinned memory (lazy)
     unsigned hseq = 0;          // last value queued for it
+    std::vector<hipEvent_t> pev;  // completion of the pieces of a pipelined copy out (lazy)
     hipStream_t stream = nullptr;
     bool own_stream = true;
     Tune sched;               // launch schedule (autotuned for large plans)

@@ -48,6 +48,8 @@ struct TileArgs {
                                        // share one wavefront when TPT <= 64
     long long pack_in, pack_out;       // packed-pair tiles: item offset of column
                                        // H-1 from column 0 (tile_kernel PACK)
+    unsigned* hflag;                   // one-work-group launches: host-polled completion
+    unsigned hval;                     // word (pinned memory) and the value to store there
 };
 
 struct TransformRef {
@@ -557,7 +559,7 @@ template <typename T, int E> constexpr int tile_threads() { return E >= 32 ? 512
 // non-temporal accesses (passes over arrays larger than the MALL, which
 // stream through it anyway).
 template <typename T, int LOGL, int E, int KIND, bool PACK = false, bool NT = false>
-__global__ void __launch_bounds__((tile_threads<T, E>())) tile_kernel(TileArgs A) {
+__device__ __forceinline__ void tile_body(const TileArgs& A) {
     using C2 = c2<T>;
     constexpr int L = 1 << LOGL;
     constexpr int TPT = L / E;
@@ -786,6 +788,23 @@ __global__ void __launch_bounds__((tile_threads<T, E>())) tile_kernel(TileArgs A
     }
 }
 
+// Completion word of a one-work-group launch, for a host that polls it
+// instead of waiting for the stream (the synchronous API's small calls):
+// every wavefront makes its stores visible system-wide (L2 written back,
+// stores acknowledged), the work-group synchronises, and one lane stores the
+// value.  Only launched with a grid of one work-group.
+__device__ __forceinline__ void signal_host(const TileArgs& A) {
+    __threadfence_system();
+    __syncthreads();
+    if (threadIdx.x == 0) __hip_atomic_store(A.hflag, A.hval, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+template <typename T, int LOGL, int E, int KIND, bool PACK = false, bool NT = false>
+__global__ void __launch_bounds__((tile_threads<T, E>())) tile_kernel(TileArgs A) {
+    tile_body<T, LOGL, E, KIND, PACK, NT>(A);
+    if (A.hflag) signal_host(A);
+}
+
 // ---------------------------------------------------------------------------
 // Element-wise kernels for the two-step path of the real kinds and for
 // length-1 axes.  One thread per item; same addressing as the tile kernel
@@ -818,22 +837,27 @@ __global__ void __launch_bounds__(256) pre_kernel(TileArgs A, int L, long long t
     out[(long long)b * L + p] = pre_point<T, KIND>(p, L, tw, X, Z);
 }
 
-// post: complex FFT output (contiguous, transform b at b*L) -> kind's output
+// post: complex FFT output (contiguous, transform b at b*L) -> kind's output.
+// The post-twiddle is computed in FP64 from the FP64 master table whatever
+// the data precision: the FFT outputs are widened, combined with exact FP64
+// twiddles and rounded once at the store (the FP32 table's rounding would
+// otherwise add to the error of these long transforms).
 template <typename T, int KIND>
 __global__ void __launch_bounds__(256) post_kernel(TileArgs A, int L, long long total) {
     using C2 = c2<T>;
-    constexpr int NO = 1;  // placeholder to keep the signature uniform
-    (void)NO;
     const int nout = kind_nout(KIND, L);
     const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= total) return;
     const unsigned b = (unsigned)(g / nout);
     const int i = (int)(g - (long long)b * nout);
     const TransformRef r = locate(A, b);
-    const C2* __restrict__ tw = reinterpret_cast<const C2*>(A.tw);
     const C2* __restrict__ src = reinterpret_cast<const C2*>(A.in) + (long long)b * L;
-    auto Tf = [&](int p) -> C2 { return src[p]; };
-    C2 y = post_item<T, KIND>(i, L, tw, Tf);
+    auto Tf = [&](int p) -> double2 {
+        const C2 v = src[p];
+        return make_double2((double)v.x, (double)v.y);
+    };
+    const double2 yd = post_item<double, KIND>(i, L, A.tw64, Tf);
+    const C2 y = mk<T>((T)yd.x, (T)yd.y);
     if constexpr (kind_complex_out(KIND)) {
         reinterpret_cast<C2*>(A.out)[r.ob + (long long)i * A.out_e] = y;
     } else {

@@ -438,3 +438,37 @@ def test_aux_stream_ordered_after_default_stream(libs, oracle64):
         for out in (y, z):
             assert abs(complex(out[0].item()) - n) <= 1e-3 * n
             assert float(out[1:].abs().max()) <= 1e-2
+
+
+def test_host_arrays_pipelined_bounce(libs, oracle64, monkeypatch):
+    """Host arrays of 4-64 MiB go through the aux's pinned bounce buffer in
+    4 MiB pieces (the host copy of a piece overlapping the DMA of the one
+    before): a 2048x2048 FP64 real DFT (32 MiB in, 2048x1025 complex out --
+    not a whole number of pieces) and a 2^21-point FP64 complex DFT, with the
+    pipeline on and off (MINFFT_HOST_PIPE=0), give identical results, which
+    match the oracle."""
+    rng = np.random.default_rng(5)
+    n = 2048
+    x = rng.standard_normal((n, n))
+    c = rng.uniform(-1, 1, 1 << 21) + 1j * rng.uniform(-1, 1, 1 << 21)
+    outs = {}
+    for pipe in ("1", "0"):
+        monkeypatch.setenv("MINFFT_HOST_PIPE", pipe)
+        print(f"[case] host arrays, pipelined bounce={pipe}", flush=True)
+        z = np.full((n, n // 2 + 1), np.nan + 0j)
+        aux = mf.Aux(libs[64], "realdft", (n, n))
+        aux("realdft", x, z)
+        back = np.empty_like(x)
+        aux("invrealdft", z, back)
+        aux.close()
+        y = np.full_like(c, np.nan)
+        aux = mf.Aux(libs[64], "dft", (1 << 21,))
+        aux("dft", c, y)
+        aux.close()
+        outs[pipe] = (z, back, y)
+    for a, b in zip(outs["1"], outs["0"]):
+        assert np.array_equal(a, b)
+    z, back, y = outs["1"]
+    assert rel_l2(z, oracle64.run("realdft", x, (n, n))) <= tol64((n, n))
+    assert rel_l2(back / (n * n), x) <= tol64((n, n))
+    assert rel_l2(y, oracle64.run("dft", c, (1 << 21,))) <= tol64((1 << 21,))

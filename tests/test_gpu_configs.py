@@ -245,3 +245,30 @@ def test_512_cubed_split_fp16_noisy_vs_oracle(oracle64):
     mx = float(np.max(np.abs(yh - ref)))
     print(f"  split-FP16 rel-L2 vs FP64 oracle {e:.3e}, max-abs {mx:.3e}", flush=True)
     assert e <= TOL32, e
+
+
+# ------------------------------------- long 1D real kinds, FP32 (two-step)
+@pytest.mark.parametrize("kind", ["realdft", "invrealdft", "dct2", "dst2", "dct3", "dct4", "dst4"])
+def test_long_real_kinds_fp32(lib32, oracle64, kind):
+    """FP32 real/DCT/DST transforms of 2^20 points (their half-length FFT
+    needs the two-step path; the final post-twiddle is computed in FP64)
+    against the FP64 oracle on the same float input: rel-L2 <= 1e-6, with
+    rel-L2 and max error printed."""
+    from tests.helpers import make_input
+    n = 1 << 20
+    shape = (n,)
+    if kind == "invrealdft":
+        xr = make_input("dct2", shape, 32, 31)
+        x = oracle64.run("realdft", xr.astype(np.float64), shape).astype(np.complex64)
+    else:
+        x = make_input(kind, shape, 32, 30)
+    aux = mf.Aux(lib32, mf.FAMILY[kind], shape)
+    odt = np.complex64 if kind == "realdft" else np.float32
+    y = np.empty(n // 2 + 1 if kind == "realdft" else n, dtype=odt)
+    aux(kind, x, y)
+    aux.close()
+    ref = oracle64.run(kind, x.astype(np.complex128 if kind == "invrealdft" else np.float64), shape)
+    e = rel_l2(y, ref)
+    mx = float(np.max(np.abs(y.astype(ref.dtype) - ref)) / np.max(np.abs(ref)))
+    print(f"[case] {kind} 2^20 FP32: rel-L2 {e:.3e}, max error / max |y| {mx:.3e}", flush=True)
+    assert e <= TOL32, e
