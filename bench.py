@@ -100,14 +100,15 @@ def dominant_group(sizes, xin, xout, offs, prec, steps, torch, mf):
     return n, len(idx), ms, float(len(idx) * n * esz * 2)
 
 
-def read_traffic(kernel_tag):
-    """HBM bytes per launch measured by rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
-    and scaled by a copy kernel of known size (profiles/c3_traffic.txt)."""
-    path = os.path.join(ROOT, "profiles", "c3_traffic.txt")
+def read_traffic(kernel_tag, config="c3"):
+    """HBM bytes per launch (or per transform, for the other configurations)
+    measured by rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE and scaled by copies
+    of known size (profiles/traffic.txt)."""
+    path = os.path.join(ROOT, "profiles", "traffic.txt")
     if not os.path.exists(path):
         return None
     for line in open(path):
-        if line.startswith("traffic ") and kernel_tag in line:
+        if line.startswith("traffic ") and f"config={config} " in line and kernel_tag in line:
             kv = dict(t.split("=", 1) for t in line.split()[1:] if "=" in t)
             try:
                 return float(kv["bytes_per_launch"])
@@ -118,19 +119,37 @@ def read_traffic(kernel_tag):
 
 def read_step_traffic(alg_bytes):
     """HBM bytes of one whole step of the default workload: the sum of the
-    per-launch traffic of every kernel of the step (profiles/c3_traffic.txt);
+    per-launch traffic of every kernel of the step (profiles/traffic.txt);
     None unless the profiled launches' algorithmic bytes add up to the
     step's (every launch of the step is covered)."""
-    path = os.path.join(ROOT, "profiles", "c3_traffic.txt")
+    path = os.path.join(ROOT, "profiles", "traffic.txt")
     if not os.path.exists(path):
         return None
     tot, alg = 0.0, 0.0
     for line in open(path):
-        if line.startswith("traffic "):
+        if line.startswith("traffic ") and "config=c3 " in line:
             kv = dict(t.split("=", 1) for t in line.split()[1:] if "=" in t)
             tot += float(kv["bytes_per_launch"])
             alg += float(kv["algorithmic_bytes"])
     return tot if abs(alg - alg_bytes) <= 0.01 * alg_bytes else None
+
+
+def config_copy(a, torch, mf):
+    """Calibration copies for the HBM byte counters: 512 MiB contiguous, then
+    the same source read as 64-byte and as 32-byte segments at a 128-byte
+    stride (whole 128-byte lines fetched, half / a quarter of them used)."""
+    src = torch.zeros(128 << 20, dtype=torch.float32, device="cuda")  # 512 MiB
+    dst = torch.empty_like(src)
+    v = src.view(-1, 32)  # rows of 128 bytes
+    d64 = torch.empty((v.shape[0], 16), dtype=torch.float32, device="cuda")
+    d32 = torch.empty((v.shape[0], 8), dtype=torch.float32, device="cuda")
+    for _ in range(3):
+        dst.copy_(src)
+        d64.copy_(v[:, :16])
+        d32.copy_(v[:, :8])
+    torch.cuda.synchronize()
+    return {"copies": "512 MiB contiguous; 64 B of every 128 B (256 MiB); 32 B of every 128 B (128 MiB)",
+            "bytes_read": [512 << 20, 256 << 20, 128 << 20], "bytes_written": [512 << 20, 256 << 20, 128 << 20]}
 
 
 def check_sample(xin, xout, sizes, offs, nsample=48):
@@ -407,7 +426,7 @@ def config_c2(a, torch, mf):
                   "host_arrays": host,
                   "roofline": {"bound": "hbm", "achieved": per["realdft"]["alg_hbm_gbs"], "peak": HBM_PEAK_GBS,
                                "unit": "GB/s", "frac": round(per["realdft"]["alg_hbm_gbs"] / HBM_PEAK_GBS, 4),
-                               "traffic": None, "kernel": "realdft (2 axis passes)"},
+                               "traffic": read_traffic("kind=realdft", "c2"), "kernel": "realdft (2 axis passes)"},
                   "cpu_baseline": cpu})
 
 
@@ -495,8 +514,8 @@ def config_c4(a, torch, mf):
                   "spike_closed_form_rel": spike_rel,
                   "roofline": {"bound": "hbm", "achieved": round(alg / (ms * 1e-3) / 1e9, 1),
                                "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                               "frac": round(alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4), "traffic": None,
-                               "kernel": "3 axis passes"},
+                               "frac": round(alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                               "traffic": read_traffic("kind=split-fp16", "c4"), "kernel": "3 axis passes"},
                   "cpu_baseline": cpu})
 
 
@@ -558,7 +577,7 @@ def config_c5(a, torch, mf):
                   "speedup_vs_fp64_gpu": round(ms64 / ms, 3),
                   "roofline": {"bound": "hbm", "achieved": round(alg / (ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,
                                "unit": "GB/s", "frac": round(alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-                               "traffic": None, "kernel": "2 axis passes per image"},
+                               "traffic": read_traffic("kind=fp32", "c5"), "kernel": "2 axis passes per image"},
                   "cpu_baseline": cpu})
 
 
@@ -571,7 +590,7 @@ def main():
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison leg")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a 512 MiB device copy (calibrates the HBM byte counters)")
-    ap.add_argument("--config", default="c3", choices=["c1", "c2", "c3", "c4", "c5"],
+    ap.add_argument("--config", default="c3", choices=["c1", "c2", "c3", "c4", "c5", "copy"],
                     help="c3 (default) is the metric's workload; the others print their own line")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -583,7 +602,7 @@ def main():
     import minfft_amd as mf
 
     if a.config != "c3":
-        fn = {"c1": config_c1, "c2": config_c2, "c4": config_c4, "c5": config_c5}[a.config]
+        fn = {"c1": config_c1, "c2": config_c2, "c4": config_c4, "c5": config_c5, "copy": config_copy}[a.config]
         print(json.dumps(fn(a, torch, mf)), flush=True)
         return
 
@@ -626,7 +645,7 @@ def main():
     step_achieved = alg_bytes / (ms32 * 1e-3) / 1e9
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
     dom_tag = f"N={dom_n}"
-    traffic = read_traffic(dom_tag)
+    traffic = read_traffic(dom_tag, "c3")
     step_traffic = read_step_traffic(alg_bytes)
     out = {
         "metric": METRIC,
