@@ -97,6 +97,15 @@ int minfft_batch_describe(const minfft_batch_plan *p, char *buf, int len);
    (also for host pointers, which the kernels cannot read). */
 int minfft_enqueue(int kind, const void *in, void *out, const void *aux, void *stream);
 
+/* Make the 27-symbol calls on `aux` (minfft_dft ... minfft_dst4) and its
+   NULL-stream enqueues run on the caller's HIP stream `stream` (hipStream_t
+   of the aux's device; NULL = back to the aux's own stream).  The calls stay
+   synchronous: they return once the result is written, and their kernels are
+   ordered after the work queued earlier on that stream.  The stream must
+   outlive its use by the aux.  Returns 0, or -1 for a bad aux or a stream of
+   another device. */
+int minfft_set_stream(const void *aux, void *stream);
+
 /* Largest scratch minfft_enqueue_noalloc may use: 64 MiB, held by each GPU
    the library runs on until the process ends. */
 #define MINFFT_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)

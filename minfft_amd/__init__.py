@@ -29,7 +29,7 @@ TRANSFORMS = [f"minfft_{k}" for k in KINDS]
 SYMBOLS = TRANSFORMS + MAKERS + ["minfft_free_aux"]
 EXT_SYMBOLS = ["minfft_last_error", "minfft_mkaux_batch", "minfft_batch_run", "minfft_batch",
                "minfft_free_batch", "minfft_batch_schedule", "minfft_batch_describe", "minfft_enqueue",
-               "minfft_enqueue_noalloc"]
+               "minfft_enqueue_noalloc", "minfft_set_stream"]
 
 PREC_FP64, PREC_FP32, PREC_SPLIT_FP16 = 0, 1, 2
 
@@ -79,6 +79,8 @@ def engine():
         lib.minfft_enqueue_noalloc.restype = ctypes.c_int
         lib.minfft_enqueue_noalloc.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
                                                ctypes.c_void_p, ctypes.c_void_p]
+        lib.minfft_set_stream.restype = ctypes.c_int
+        lib.minfft_set_stream.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
         _ENGINE = lib
     return _ENGINE
 
@@ -179,6 +181,14 @@ class Aux:
         if rc:
             raise MinfftError(f"minfft_enqueue_noalloc({kind}): {last_error()}")
         return y
+
+    def set_stream(self, stream=None):
+        """minfft_set_stream (minfft_ext.h): the synchronous calls and
+        NULL-stream enqueues of this aux run on `stream` (a hipStream_t
+        handle; None = the aux's own stream)."""
+        rc = engine().minfft_set_stream(self.h, stream)
+        if rc:
+            raise MinfftError(f"minfft_set_stream: {last_error()}")
 
     def close(self):
         if self.h:

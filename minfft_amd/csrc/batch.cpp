@@ -152,6 +152,30 @@ int minfft_enqueue(int kind, const void* in, void* out, const void* aux, void* s
     return enqueue(*p, kind, in, out, st);
 }
 
+int minfft_set_stream(const void* aux, void* stream) {
+    clear_error();
+    Plan* p = reinterpret_cast<Plan*>(const_cast<void*>(aux));
+    if (!p || p->magic != 0x6d667870u) {
+        set_error("invalid aux data");
+        return -1;
+    }
+    if (!stream) {
+        p->stream = p->home;
+        return 0;
+    }
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    // a stream of another GPU would run the aux's kernels on memory that GPU
+    // does not hold
+    hipDevice_t sd = -1;
+    if (hipStreamGetDevice(st, &sd) != hipSuccess || sd != p->dev) {
+        (void)hipGetLastError();
+        set_error("stream does not belong to the aux's device");
+        return -1;
+    }
+    p->stream = st;
+    return 0;
+}
+
 void minfft_free_batch(minfft_batch_plan* p) {
     if (!p) return;
     if (p->exec) (void)hipGraphExecDestroy(p->exec);

@@ -942,7 +942,7 @@ static void family_kinds(Family f, int* k, int& n) {
 
 void free_plan(Plan* p) {
     if (!p) return;
-    if (p->stream && p->own_stream) (void)hipStreamDestroy(p->stream);
+    if (p->home) (void)hipStreamDestroy(p->home);
     free_workspace(p->ws);
     if (p->tbuf) (void)hipFree(p->tbuf);
     if (p->stage) (void)hipFree(p->stage);
@@ -989,8 +989,9 @@ Plan* make_plan(Family fam, Prec prec, int d, const int* Ns) {
     // a blocking stream: its work is ordered after work queued earlier on the
     // legacy default stream, so a synchronous call on device arrays sees what
     // the caller's default-stream kernels wrote into them
-    if (hipStreamCreateWithFlags(&p->stream, hipStreamDefault) != hipSuccess)
+    if (hipStreamCreateWithFlags(&p->home, hipStreamDefault) != hipSuccess)
         return fail("hipStreamCreate failed");
+    p->stream = p->home;
     const size_t rs = real_size(prec), cs = 2 * rs;
     // twiddle range and scratch over all axis passes of the family
     int kinds[2], nk;

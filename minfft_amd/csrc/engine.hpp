@@ -121,8 +121,8 @@ struct Plan {
     unsigned* hflag = nullptr;  // completion word in coherent pinned memory (lazy)
     unsigned hseq = 0;          // last value queued for it
     std::vector<hipEvent_t> pev;  // completion of the pieces of a pipelined copy out (lazy)
-    hipStream_t stream = nullptr;
-    bool own_stream = true;
+    hipStream_t stream = nullptr;  // stream of synchronous calls and NULL-stream enqueues
+    hipStream_t home = nullptr;    // the plan's own stream (made with the plan)
     Tune sched;               // launch schedule (autotuned for large plans)
     bool tuned = false;
 };

@@ -440,6 +440,45 @@ def test_aux_stream_ordered_after_default_stream(libs, oracle64):
             assert float(out[1:].abs().max()) <= 1e-2
 
 
+def test_set_stream_runs_calls_on_caller_stream(libs):
+    """minfft_set_stream: after it, the aux's synchronous calls and NULL-stream
+    enqueues are ordered after work queued on the caller's (non-blocking)
+    stream -- an input written there by a long chain of kernels just before
+    the call is what the transform reads -- and the synchronous call has
+    written its result on return.  NULL restores the aux's own stream; a bad
+    aux is refused."""
+    import torch
+    n = 1 << 20
+    s = torch.cuda.Stream()
+    for prec in (32, 64):
+        cdt = torch.complex64 if prec == 32 else torch.complex128
+        aux = mf.Aux(libs[prec], "dft", (n,))
+        print(f"[case] set_stream prec={prec}", flush=True)
+        aux.set_stream(s.cuda_stream)
+        x = torch.empty(n, dtype=cdt, device="cuda")
+        y = torch.full_like(x, float("nan"))
+        z = torch.full_like(x, float("nan"))
+        torch.cuda.synchronize()
+        with torch.cuda.stream(s):
+            x.zero_()
+            for k in range(64):  # ~64 kernels still queued on s when the calls are made
+                x.add_(1.0 / 64)
+        aux.enqueue("dft", x, y)  # NULL stream: now s
+        aux("dft", x, z)          # synchronous, on s
+        zc = z.cpu()              # no synchronization of s before this read
+        torch.cuda.synchronize()
+        for out in (y.cpu(), zc):
+            assert abs(complex(out[0].item()) - n) <= 1e-3 * n
+            assert float(out[1:].abs().max()) <= 1e-2
+        aux.set_stream(None)
+        w = torch.empty_like(x)
+        aux("dft", x, w)
+        assert torch.equal(w, z)
+        aux.close()
+    rc = mf.engine().minfft_set_stream(None, s.cuda_stream)
+    assert rc == -1 and "invalid aux" in mf.last_error()
+
+
 def test_host_arrays_pipelined_bounce(libs, oracle64, monkeypatch):
     """Host arrays of 4-64 MiB go through the aux's pinned bounce buffer in
     4 MiB pieces (the host copy of a piece overlapping the DMA of the one
