@@ -5,7 +5,13 @@ Produces, under minfft_amd/lib/:
   libminfft.so         the 27 minfft.h entry points, double precision
   libminfft_single.so  the same entry points, single precision (MINFFT_SINGLE)
 
-Usage: python -m minfft_amd.build [--force] [-j N]
+and, with --sanitize, under minfft_amd/lib/san/ the same three libraries with
+their host code (plans, batch, tuner, staging, API) instrumented by
+AddressSanitizer and UndefinedBehaviorSanitizer (-Xarch_host: the kernels
+are the ordinary objects), plus the driver programs tests/c_app/host_checks.cpp
+linked against them (host_checks64, host_checks32).
+
+Usage: python -m minfft_amd.build [--force] [-j N] [--sanitize]
 """
 import argparse
 import os
@@ -30,6 +36,14 @@ COMMON = [
     "-I", INC, "-I", CSRC,
     "-Wall", "-Wno-unused-function", "-Wno-unused-variable", "-Wno-unused-but-set-variable",
 ]
+
+# host-only instrumentation: each -fsanitize= applies to the host pass only
+SAN_FLAGS = ["-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fsanitize=undefined",
+             "-Xarch_host", "-fno-omit-frame-pointer", "-g"]
+SAN_OBJ = os.path.join(OBJ, "san")
+SAN_LIB = os.path.join(LIB, "san")
+CLANGXX = os.environ.get("MINFFT_CLANGXX", "/opt/rocm/llvm/bin/clang++")
+ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 
 HEADERS = ["fft_core.hpp", "kinds.hpp", "tile.hpp", "ktable.hpp", "engine.hpp", "big.hpp", "bigtable.hpp", "mfma.hpp",
            "autotune.hpp", "split2.hpp", "rsplit.hpp", "multi.hpp", "coop.hpp"]
@@ -60,6 +74,12 @@ def _units():
     u.append((os.path.join(OBJ, "api64.o"), "api.cpp", ["-x", "hip"], "api64"))
     u.append((os.path.join(OBJ, "api32.o"), "api.cpp", ["-x", "hip", "-DMINFFT_SINGLE=1"], "api32"))
     return u
+
+
+def _san_units(units):
+    """The host-code units again, instrumented, into _build/san/."""
+    return [(os.path.join(SAN_OBJ, os.path.basename(o)), src, extra + SAN_FLAGS, tag)
+            for o, src, extra, tag in units if src.endswith(".cpp")]
 
 
 def _compile(unit, force):
@@ -110,12 +130,65 @@ def build(force=False, jobs=None, verbose=True):
     return LIB
 
 
+def build_sanitized(force=False, jobs=None, verbose=True):
+    """The sanitizer variant of the libraries (after build(): it reuses the
+    kernel objects) and the host_checks driver programs, in lib/san/.  The
+    libraries leave the sanitizer runtimes' symbols to the program that loads
+    them, which links the runtimes itself (clang++ -fsanitize=...)."""
+    build(force=False, jobs=jobs, verbose=False)
+    os.makedirs(SAN_OBJ, exist_ok=True)
+    os.makedirs(SAN_LIB, exist_ok=True)
+    units = _units()
+    san = _san_units(units)
+    jobs = jobs or min(8, os.cpu_count() or 4)
+    errors = []
+    with ThreadPoolExecutor(max_workers=jobs) as ex:
+        for obj, err in ex.map(lambda u: _compile(u, force), san):
+            if err:
+                errors.append(err)
+    if errors:
+        raise RuntimeError("hipcc failed:\n" + "\n".join(errors))
+    by_src = {(u[1], u[3]): u[0] for u in san}
+    eng = [by_src.get((u[1], u[3]), u[0]) for u in units if u[3] == "engine"]
+    eng_so = os.path.join(SAN_LIB, "libminfft_engine.so")
+    if force or _newer(eng_so, eng):
+        _link(eng_so, eng)
+    rpath = ["-Wl,-rpath,$ORIGIN", "-L" + SAN_LIB, "-lminfft_engine"]
+    for name, tag in (("libminfft.so", "api64"), ("libminfft_single.so", "api32")):
+        objs = [u[0] for u in san if u[3] == tag]
+        so = os.path.join(SAN_LIB, name)
+        if force or _newer(so, objs + [eng_so]):
+            _link(so, objs, rpath)
+    src = os.path.join(ROOT, "tests", "c_app", "host_checks.cpp")
+    for single in (0, 1):
+        exe = os.path.join(SAN_LIB, f"host_checks{32 if single else 64}")
+        deps = [src, os.path.join(INC, "minfft.h"), os.path.join(INC, "minfft_ext.h"), eng_so]
+        if not force and not _newer(exe, deps):
+            continue
+        cmd = [CLANGXX, "-std=c++17", "-O1", "-g", "-Wall", "-Wextra", "-Werror",
+               "-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fsanitize=undefined",
+               "-fno-omit-frame-pointer", "-D__HIP_PLATFORM_AMD__", f"-DMINFFT_SINGLE={single}",
+               "-I", INC, "-I", os.path.join(ROCM, "include"), src, "-o", exe,
+               "-L" + SAN_LIB, "-lminfft_single" if single else "-lminfft", "-lminfft_engine",
+               "-L" + os.path.join(ROCM, "lib"), "-lamdhip64", "-lpthread",
+               f"-Wl,-rpath,{SAN_LIB}", f"-Wl,-rpath,{os.path.join(ROCM, 'lib')}"]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"$ {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    if verbose:
+        print(f"built the sanitizer variant in {SAN_LIB}")
+    return SAN_LIB
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--force", action="store_true")
     ap.add_argument("-j", type=int, default=None)
+    ap.add_argument("--sanitize", action="store_true")
     a = ap.parse_args(argv)
     build(force=a.force, jobs=a.j)
+    if a.sanitize:
+        build_sanitized(force=a.force, jobs=a.j)
 
 
 if __name__ == "__main__":
