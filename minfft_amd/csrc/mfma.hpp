@@ -3,34 +3,31 @@
 // batch extension, MINFFT_SPLIT_FP16).
 //
 // L = 512 = 16 * 16 * 2, Stockham passes (Ns = 1, 16, 256):
-//   pass 1, radix 16 on MFMA: for a group of 16 butterflies (columns j) the
-//     16 inputs x_r (rows r) form a 16x16 complex block; X = F16 * x with
-//     F16[k][r] = exp(-+2 pi i k r / 16), in real arithmetic
+//   pass 1, radix 16 on MFMA: for a group of 16 columns (butterflies, or
+//     transforms) the 16 inputs x_r (rows r) form a 16x16 complex block;
+//     X = F16 * x with F16[k][r] = exp(-+2 pi i k r / 16), in real arithmetic
 //         Xr = C*xr + S*xi,  Xi = C*xi - S*xr     (forward; S -> -S inverse)
-//     C = cos, S = sin of 2 pi k r / 16.  Every operand is split into FP16
-//     hi + lo parts (both rounded to nearest) and the product kept to ~22
-//     bits as
-//         A*x ~ A_hi*x_hi + A_hi*x_lo + A_lo*x_hi
-//     with v_mfma_f32_16x16x16f16 accumulating in FP32 (12 MFMAs per
-//     group).  Each transform is scaled by a power of two first, so that its
-//     largest component lies in [2^13, 2^14): FP16 never overflows and the
-//     error stays relative to the transform's own magnitude; the scale is
-//     undone exactly on the FP32 result.
+//     C = cos, S = sin of 2 pi k r / 16, i.e. Xr = [C | S] [xr ; xi] and
+//     Xi = [-S | C] [xr ; xi]: one K = 32 product each.  Every operand is
+//     split into FP16 hi + lo parts (both rounded to nearest) and the
+//     product kept to ~22 bits as
+//         A*x ~ A_lo*x_hi + A_hi*x_lo + A_hi*x_hi
+//     with v_mfma_f32_16x16x32_f16 accumulating in FP32 (6 MFMAs per
+//     group).  The matrix parts are compile-time constants; the data are
+//     scaled by a power of two first, so that the largest component lies in
+//     [2^13, 2^14): FP16 never overflows and the error stays relative to the
+//     data's own magnitude; the scale is undone exactly on the FP32 result.
 //   passes 2 and 3 (radix 16 with twiddles W_256, radix 2 with W_512) run in
 //     FP32 on the vector ALUs, as in fft_core.hpp.
-// Each wavefront owns two transforms.  Butterflies 2c and 2c+1 (c < 16) form
-// the two column groups of a transform, so that lane (c, g) reads the
-// adjacent inputs x[2c + 32r], x[2c + 1 + 32r] (r = 4g + q) as one 16-byte
-// load straight from memory for contiguous rows; the LDS exchanges after
-// pass 1 stay inside the wavefront.
-// Operand layouts of v_mfma_f32_16x16x16f16 (lane l, element q = 0..3):
-//   A[l%16][4*(l/16)+q], B[4*(l/16)+q][l%16], D[4*(l/16)+q][l%16].
+// Operand layouts of v_mfma_f32_16x16x32_f16 (lane l, element j = 0..7,
+// q = 0..3): A[l%16][8*(l/16)+j], B[8*(l/16)+j][l%16], D[4*(l/16)+q][l%16].
 #pragma once
 #include "tile.hpp"
 
 namespace mfx {
 
 typedef _Float16 half4_t __attribute__((ext_vector_type(4)));
+typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
 typedef float float4_t __attribute__((ext_vector_type(4)));
 typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
 typedef float float2_t __attribute__((ext_vector_type(2)));
@@ -42,9 +39,59 @@ __device__ __forceinline__ float4 ld4_nt(const float2* p) {
     return make_float4(v.x, v.y, v.z, v.w);
 }
 
-__device__ __forceinline__ void split_h(float x, _Float16& hi, _Float16& lo) {
-    hi = (_Float16)x;
-    lo = (_Float16)(x - (float)hi);
+// DFT-16 matrix operands of every lane of a wavefront for
+// v_mfma_f32_16x16x32_f16, split into FP16 hi + lo parts at compile time from
+// the exact cosines of multiples of pi/8.  Lane (k, g) = (lane % 16, lane / 16)
+// holds A[k][8g + j], j < 8, of the 16x32 blocks Ar = [C | Pr] and
+// Ai = [Pi | C] that multiply the 32-row operand [x_r ; x_i]:
+//   C  = cos(2 pi k r / 16),
+//   Pr = sin (forward) or -sin (inverse) -- multiplies x_i into X_r,
+//   Pi = -Pr -- multiplies x_r into X_i.
+// Row s = 8g + j of the operand is x_r (g < 2) or x_i (g >= 2) of point
+// r = 4h + j + 4 [j >= 4], h = g % 2: the points that lanes (c, h) and
+// (c, h + 2) load as r = 4g + q, once their real and imaginary parts are
+// swapped between the two halves of the wavefront (mfma32_operand_b).
+struct alignas(16) Mfma32Ops {
+    _Float16 h[32];  // arh[8] arl[8] aih[8] ail[8]
+};
+struct Mfma32Table {
+    Mfma32Ops e[2][64];  // [inverse][lane]
+};
+constexpr double kCosPi8[16] = {1.0, 0.92387953251128675613, 0.70710678118654752440, 0.38268343236508977173,
+                                0.0, -0.38268343236508977173, -0.70710678118654752440, -0.92387953251128675613,
+                                -1.0, -0.92387953251128675613, -0.70710678118654752440, -0.38268343236508977173,
+                                0.0, 0.38268343236508977173, 0.70710678118654752440, 0.92387953251128675613};
+constexpr Mfma32Table make_mfma32_table() {
+    Mfma32Table t{};
+    for (int inv = 0; inv < 2; ++inv)
+        for (int lane = 0; lane < 64; ++lane) {
+            const int k = lane & 15, g = lane >> 4;
+            for (int j = 0; j < 8; ++j) {
+                const int r = 4 * (g & 1) + j + (j >= 4 ? 4 : 0);
+                const int m = (k * r) & 15;
+                const double c = kCosPi8[m], sn = kCosPi8[(m + 12) & 15];  // cos, sin of pi m / 8
+                const double pr = inv ? -sn : sn;
+                const double ar = g < 2 ? c : pr, ai = g < 2 ? -pr : c;
+                const _Float16 arh = (_Float16)ar, aih = (_Float16)ai;
+                _Float16* h = t.e[inv][lane].h;
+                h[j] = arh;
+                h[8 + j] = (_Float16)(ar - (double)arh);
+                h[16 + j] = aih;
+                h[24 + j] = (_Float16)(ai - (double)aih);
+            }
+        }
+    return t;
+}
+__device__ const Mfma32Table kMfma32 = make_mfma32_table();
+
+struct Mfma32Operands {
+    half8_t arh, arl, aih, ail;
+};
+// this lane's operands (four 16-byte loads of an 8 KiB table)
+template <bool INV>
+__device__ __forceinline__ Mfma32Operands mfma32_operands(int lane) {
+    const half8_t* p = reinterpret_cast<const half8_t*>(kMfma32.e[INV ? 1 : 0][lane].h);
+    return Mfma32Operands{p[0], p[1], p[2], p[3]};
 }
 
 // largest value over the 64 lanes of the wavefront (every lane gets it)
@@ -54,20 +101,65 @@ __device__ __forceinline__ float wave_max(float m) {
     return m;
 }
 
+// -1.0f in a register the compiler does not see as a constant, so that
+// fma(hi, m1, x) stays an fma with an FP16 operand (one v_fma_mix_f32)
+// instead of being folded into a conversion and a subtraction
+__device__ __forceinline__ float opaque_m1() {
+    float m1;
+    asm volatile("s_mov_b32 %0, -1.0" : "=s"(m1));
+    return m1;
+}
+
 // hi + lo FP16 split of four FP32 values (round to nearest for both parts:
-// x = hi + lo to ~22 bits), packed as MFMA operands
-__device__ __forceinline__ void split4(float a0, float a1, float a2, float a3, half4_t& hi, half4_t& lo) {
+// x = hi + lo to ~22 bits), packed as MFMA operands; m1 = opaque_m1()
+__device__ __forceinline__ void split4(float a0, float a1, float a2, float a3, float m1, half4_t& hi, half4_t& lo) {
     const half2_t h01 = __builtin_convertvector((float2_t){a0, a1}, half2_t);
     const half2_t h23 = __builtin_convertvector((float2_t){a2, a3}, half2_t);
-    const half2_t l01 = __builtin_convertvector((float2_t){a0 - (float)h01.x, a1 - (float)h01.y}, half2_t);
-    const half2_t l23 = __builtin_convertvector((float2_t){a2 - (float)h23.x, a3 - (float)h23.y}, half2_t);
+    // x - hi is exact in FP32; one rounding to FP16
+    const half2_t l01 = __builtin_convertvector(
+        (float2_t){__builtin_fmaf((float)h01.x, m1, a0), __builtin_fmaf((float)h01.y, m1, a1)}, half2_t);
+    const half2_t l23 = __builtin_convertvector(
+        (float2_t){__builtin_fmaf((float)h23.x, m1, a2), __builtin_fmaf((float)h23.y, m1, a3)}, half2_t);
     hi = __builtin_shufflevector(h01, h23, 0, 1, 2, 3);
     lo = __builtin_shufflevector(l01, l23, 0, 1, 2, 3);
+}
+
+// The 32-row operand [x_r ; x_i] of a column (hi and lo parts) from lane
+// (c, g)'s points r = 4g + q: lanes of the lower half of the wavefront keep
+// their real parts and take those of lane + 32, lanes of the upper half keep
+// their imaginary parts and take those of lane - 32 (v_permlane32_swap)
+__device__ __forceinline__ void mfma32_operand_b(const float (&xr)[4], const float (&xi)[4], float m1, half8_t& bh,
+                                                 half8_t& bl) {
+    float u[4], v[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const auto sw = __builtin_amdgcn_permlane32_swap(__float_as_uint(xr[q]), __float_as_uint(xi[q]), false, false);
+        u[q] = __uint_as_float(sw[0]);
+        v[q] = __uint_as_float(sw[1]);
+    }
+    half4_t uh, ul, vh, vl;
+    split4(u[0], u[1], u[2], u[3], m1, uh, ul);
+    split4(v[0], v[1], v[2], v[3], m1, vh, vl);
+    bh = __builtin_shufflevector(uh, vh, 0, 1, 2, 3, 4, 5, 6, 7);
+    bl = __builtin_shufflevector(ul, vl, 0, 1, 2, 3, 4, 5, 6, 7);
+}
+
+// A*b to ~22 bits from hi/lo parts (lo*lo dropped), smallest terms first
+__device__ __forceinline__ float4_t mfma32_product(half8_t ah, half8_t al, half8_t bh, half8_t bl) {
+    float4_t d = {0.f, 0.f, 0.f, 0.f};
+    d = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bh, d, 0, 0, 0);
+    d = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl, d, 0, 0, 0);
+    d = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bh, d, 0, 0, 0);
+    return d;
 }
 
 // C = A.C transforms per work-group (even, at most 16), 32 lanes each; every
 // wavefront owns two whole transforms (2w, 2w+1), so apart from staged
 // column I/O the work-group's wavefronts never wait on each other.
+// Butterflies 2c and 2c+1 (c < 16) form the two column groups of a
+// transform, so that lane (c, g) reads the adjacent inputs x[2c + 32r],
+// x[2c + 1 + 32r] (r = 4g + q) as one 16-byte load straight from memory for
+// contiguous rows; the LDS exchanges after pass 1 stay inside the wavefront.
 // NT: direct I/O with non-temporal accesses (passes larger than the MALL)
 template <bool INV, bool NT = false>
 __global__ void __launch_bounds__(512) mfma512_kernel(TileArgs A) {
@@ -87,12 +179,10 @@ __global__ void __launch_bounds__(512) mfma512_kernel(TileArgs A) {
     C2* slot = lds + (long long)c * A.sc;
     C2* wslot = lds + (long long)(2 * w) * A.sc;  // the wavefront's first slot
 
-    // ---- DFT-16 matrix operands A[k][r] = W_16^{kr}, k = cc, r = 4g + q, from
-    // the twiddle table (tw[16 + m] = exp(-2 pi i m/16)); read before the
-    // inputs so that waiting for them never waits for the inputs ----
-    C2 wk[4];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) wk[q] = twl<float>(tw, 16u + (unsigned)((cc * (4 * g + q)) & 15));
+    // ---- DFT-16 matrix operands from the compile-time table, read before
+    // the inputs so that waiting for them never waits for the inputs ----
+    const Mfma32Operands op = mfma32_operands<INV>(lane);
+    const float m1 = opaque_m1();
 
     // ---- pass-1 inputs in MFMA operand layout: lane (cc, g) holds, for each
     // of the wavefront's transforms tt and q < 4, the pair
@@ -166,23 +256,7 @@ __global__ void __launch_bounds__(512) mfma512_kernel(TileArgs A) {
         sh[tt] = __builtin_amdgcn_readfirstlane((m > 0.f) ? 14 - ex : 0);
     }
 
-    half4_t Ch, Cl, Sh, Sl, NSh, NSl;
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        _Float16 h, lo;
-        split_h(wk[q].x, h, lo);  // cos
-        Ch[q] = h;
-        Cl[q] = lo;
-        split_h(-wk[q].y, h, lo);  // sin
-        Sh[q] = h;
-        Sl[q] = lo;
-        NSh[q] = -h;
-        NSl[q] = -lo;
-    }
     // forward: Xr = C xr + S xi, Xi = C xi - S xr; inverse flips S
-    const half4_t Pr_h = INV ? NSh : Sh, Pr_l = INV ? NSl : Sl;  // multiplies xi into Xr
-    const half4_t Pi_h = INV ? Sh : NSh, Pi_l = INV ? Sl : NSl;  // multiplies xr into Xi
-
     // ---- pass 1: radix 16 on the matrix cores (Ns = 1, no twiddles); output
     // k = 4g + q of butterfly j = 2cc + h goes to position 16j + k ----
 #pragma unroll
@@ -196,22 +270,10 @@ __global__ void __launch_bounds__(512) mfma512_kernel(TileArgs A) {
                 xr[q] = ldexpf(h ? P[tt][q].z : P[tt][q].x, sh[tt]);
                 xi[q] = ldexpf(h ? P[tt][q].w : P[tt][q].y, sh[tt]);
             }
-            half4_t xrh, xrl, xih, xil;
-            split4(xr[0], xr[1], xr[2], xr[3], xrh, xrl);
-            split4(xi[0], xi[1], xi[2], xi[3], xih, xil);
-            float4_t Xr = {0.f, 0.f, 0.f, 0.f}, Xi = {0.f, 0.f, 0.f, 0.f};
-            Xr = __builtin_amdgcn_mfma_f32_16x16x16f16(Cl, xrh, Xr, 0, 0, 0);
-            Xr = __builtin_amdgcn_mfma_f32_16x16x16f16(Ch, xrl, Xr, 0, 0, 0);
-            Xr = __builtin_amdgcn_mfma_f32_16x16x16f16(Pr_l, xih, Xr, 0, 0, 0);
-            Xr = __builtin_amdgcn_mfma_f32_16x16x16f16(Pr_h, xil, Xr, 0, 0, 0);
-            Xr = __builtin_amdgcn_mfma_f32_16x16x16f16(Ch, xrh, Xr, 0, 0, 0);
-            Xr = __builtin_amdgcn_mfma_f32_16x16x16f16(Pr_h, xih, Xr, 0, 0, 0);
-            Xi = __builtin_amdgcn_mfma_f32_16x16x16f16(Cl, xih, Xi, 0, 0, 0);
-            Xi = __builtin_amdgcn_mfma_f32_16x16x16f16(Ch, xil, Xi, 0, 0, 0);
-            Xi = __builtin_amdgcn_mfma_f32_16x16x16f16(Pi_l, xrh, Xi, 0, 0, 0);
-            Xi = __builtin_amdgcn_mfma_f32_16x16x16f16(Pi_h, xrl, Xi, 0, 0, 0);
-            Xi = __builtin_amdgcn_mfma_f32_16x16x16f16(Ch, xih, Xi, 0, 0, 0);
-            Xi = __builtin_amdgcn_mfma_f32_16x16x16f16(Pi_h, xrh, Xi, 0, 0, 0);
+            half8_t bh, bl;
+            mfma32_operand_b(xr, xi, m1, bh, bl);
+            const float4_t Xr = mfma32_product(op.arh, op.arl, bh, bl);
+            const float4_t Xi = mfma32_product(op.aih, op.ail, bh, bl);
             // position 16(2cc + h) + 4g + q: padded 34cc + 17h + 4g + q
             C2* d = s + (34 * cc + 17 * h + 4 * g);
 #pragma unroll
@@ -260,8 +322,9 @@ __global__ void __launch_bounds__(512) mfma512_kernel(TileArgs A) {
 // than 16 butterflies.  Group (butterfly j, transforms c < 16): lane (c, g)
 // reads x_c[j + 32r], r = 4g + q -- for fixed (g, q) the 16 lanes read 16
 // adjacent points, 128 contiguous bytes -- and wavefront w takes the
-// butterflies j = w + 8i, i < 4.  The scale is per column (transform c,
-// butterfly j): its 16 inputs sit in lanes c + 16g.  Passes 2 and 3 run
+// butterflies j = 4w + i, i < 4.  One scale serves the wavefront's four
+// columns of transform c (their 64 inputs sit in lanes c + 16g).  Passes 2
+// and 3 run
 // with the c-fast core mapping (lane t: transform t % 16, thread t / 16),
 // whose owned positions give coalesced direct stores.
 template <bool INV, bool NT = false>
@@ -279,22 +342,21 @@ __global__ void __launch_bounds__(512) mfma512_cf_kernel(TileArgs A) {
     const TransformRef me = locate(A, tile * (unsigned)CT + (unsigned)c);
     C2* slot = lds + (long long)c * A.sc;
 
-    // DFT-16 matrix operands A[k][r] = W_16^{kr}, k = c, r = 4g + q
-    C2 wk[4];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) wk[q] = twl<float>(tw, 16u + (unsigned)((c * (4 * g + q)) & 15));
+    // DFT-16 matrix operands from the compile-time table
+    const Mfma32Operands op = mfma32_operands<INV>(t & 63);
+    const float m1 = opaque_m1();
 
-    // inputs x_c[j + 32(4g + q)] of the wavefront's butterflies j = w + 8i
+    // inputs x_c[j + 32(4g + q)] of the wavefront's butterflies j = 4w + i
     C2 x[4][4];
     if (me.valid) {
-        // element j + 32(4g + q) = (w + 128g) + 8i + 32q: one base, immediate steps
+        // element j + 32(4g + q) = (4w + 128g) + i + 32q: one base, immediate steps
         const long long es = A.in_e;
-        const C2* __restrict__ p0 = reinterpret_cast<const C2*>(A.in) + me.ib + (long long)(w + 128 * g) * es;
+        const C2* __restrict__ p0 = reinterpret_cast<const C2*>(A.in) + me.ib + (long long)(4 * w + 128 * g) * es;
 #pragma unroll
         for (int i = 0; i < 4; ++i)
 #pragma unroll
             for (int q = 0; q < 4; ++q)
-                x[i][q] = NT ? ld_nt<float>(&p0[(long long)(8 * i + 32 * q) * es]) : p0[(long long)(8 * i + 32 * q) * es];
+                x[i][q] = NT ? ld_nt<float>(&p0[(long long)(i + 32 * q) * es]) : p0[(long long)(i + 32 * q) * es];
     } else {
 #pragma unroll
         for (int i = 0; i < 4; ++i)
@@ -302,52 +364,36 @@ __global__ void __launch_bounds__(512) mfma512_cf_kernel(TileArgs A) {
             for (int q = 0; q < 4; ++q) x[i][q] = make_float2(0.f, 0.f);
     }
 
-    half4_t Ch, Cl, Sh, Sl, NSh, NSl;
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        _Float16 h, lo;
-        split_h(wk[q].x, h, lo);  // cos
-        Ch[q] = h;
-        Cl[q] = lo;
-        split_h(-wk[q].y, h, lo);  // sin
-        Sh[q] = h;
-        Sl[q] = lo;
-        NSh[q] = -h;
-        NSl[q] = -lo;
-    }
-    const half4_t Pr_h = INV ? NSh : Sh, Pr_l = INV ? NSl : Sl;  // multiplies xi into Xr
-    const half4_t Pi_h = INV ? Sh : NSh, Pi_l = INV ? Sl : NSl;  // multiplies xr into Xi
-
     // ---- pass 1 on the matrix cores: output k = 4g + q of butterfly j of
-    // transform c goes to position 16j + k of slot c ----
+    // transform c goes to position 16j + k of slot c.  One scale for the
+    // wavefront's four columns of transform c (lanes c + 16g): their largest
+    // |component| -> [2^13, 2^14) ----
+    float m = 0.f;
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        const int j = w + 8 * i;
-        // column scale: largest |component| of the column -> [2^13, 2^14)
-        float m = 0.f;
+    for (int i = 0; i < 4; ++i)
 #pragma unroll
         for (int q = 0; q < 4; ++q) m = fmaxf(m, fmaxf(fabsf(x[i][q].x), fabsf(x[i][q].y)));
-        m = fmaxf(m, __shfl_xor(m, 16));
-        m = fmaxf(m, __shfl_xor(m, 32));
-        int ex = 0;
-        (void)frexpf(m, &ex);
-        const int sh = (m > 0.f) ? 14 - ex : 0;
-        half4_t xrh, xrl, xih, xil;
-        split4(ldexpf(x[i][0].x, sh), ldexpf(x[i][1].x, sh), ldexpf(x[i][2].x, sh), ldexpf(x[i][3].x, sh), xrh, xrl);
-        split4(ldexpf(x[i][0].y, sh), ldexpf(x[i][1].y, sh), ldexpf(x[i][2].y, sh), ldexpf(x[i][3].y, sh), xih, xil);
-        float4_t Xr = {0.f, 0.f, 0.f, 0.f}, Xi = {0.f, 0.f, 0.f, 0.f};
-        Xr = __builtin_amdgcn_mfma_f32_16x16x16f16(Cl, xrh, Xr, 0, 0, 0);
-        Xr = __builtin_amdgcn_mfma_f32_16x16x16f16(Ch, xrl, Xr, 0, 0, 0);
-        Xr = __builtin_amdgcn_mfma_f32_16x16x16f16(Pr_l, xih, Xr, 0, 0, 0);
-        Xr = __builtin_amdgcn_mfma_f32_16x16x16f16(Pr_h, xil, Xr, 0, 0, 0);
-        Xr = __builtin_amdgcn_mfma_f32_16x16x16f16(Ch, xrh, Xr, 0, 0, 0);
-        Xr = __builtin_amdgcn_mfma_f32_16x16x16f16(Pr_h, xih, Xr, 0, 0, 0);
-        Xi = __builtin_amdgcn_mfma_f32_16x16x16f16(Cl, xih, Xi, 0, 0, 0);
-        Xi = __builtin_amdgcn_mfma_f32_16x16x16f16(Ch, xil, Xi, 0, 0, 0);
-        Xi = __builtin_amdgcn_mfma_f32_16x16x16f16(Pi_l, xrh, Xi, 0, 0, 0);
-        Xi = __builtin_amdgcn_mfma_f32_16x16x16f16(Pi_h, xrl, Xi, 0, 0, 0);
-        Xi = __builtin_amdgcn_mfma_f32_16x16x16f16(Ch, xih, Xi, 0, 0, 0);
-        Xi = __builtin_amdgcn_mfma_f32_16x16x16f16(Pi_h, xrh, Xi, 0, 0, 0);
+    m = fmaxf(m, __shfl_xor(m, 16));
+    m = fmaxf(m, __shfl_xor(m, 32));
+    int ex = 0;
+    (void)frexpf(m, &ex);
+    const int sh = (m > 0.f) ? 14 - ex : 0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int j = 4 * w + i;
+        // the 32-row operand [x_r ; x_i]: lanes of the lower half keep their
+        // real parts and take those of lane + 32; lanes of the upper half
+        // keep their imaginary parts and take those of lane - 32
+        float xr[4], xi[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            xr[q] = ldexpf(x[i][q].x, sh);
+            xi[q] = ldexpf(x[i][q].y, sh);
+        }
+        half8_t bh, bl;
+        mfma32_operand_b(xr, xi, m1, bh, bl);
+        const float4_t Xr = mfma32_product(op.arh, op.arl, bh, bl);
+        const float4_t Xi = mfma32_product(op.aih, op.ail, bh, bl);
         // position 16j + 4g + q: padded 17j + 4g + q
         C2* d = slot + (17 * j + 4 * g);
 #pragma unroll
