@@ -430,6 +430,79 @@ def config_c2(a, torch, mf):
                   "cpu_baseline": cpu})
 
 
+def config_rlong(a, torch, mf):
+    """Long 1D real transforms whose half-length FFT exceeds one tile: 32
+    entries of 2^20 points (FP32 real DFT, DCT-2, DCT-3, DCT-4; FP64 real
+    DFT, 16 entries) as batches, two HBM passes (pre/post-processing fused
+    into the two-step FFT) against the four-pass form (schedule rlong=0)."""
+    from oracle.oracle import Oracle, Reference, ref_available
+    n = 1 << 20
+    st = torch.cuda.Stream()
+    sh = st.cuda_stream
+    o = Oracle(64)
+    res = {}
+    for prec, kind, cnt in ((32, "realdft", 32), (32, "dct2", 32), (32, "dct3", 32), (32, "dct4", 32),
+                            (64, "realdft", 16)):
+        rs = 4 if prec == 32 else 8
+        ient = n * rs
+        oent = (n // 2 + 1) * 2 * rs if kind == "realdft" else n * rs
+        stride = (max(ient, oent) + 4095) // 4096 * 4096
+        g = torch.Generator(device="cuda")
+        g.manual_seed(9)
+        dt = torch.float32 if prec == 32 else torch.float64
+        xin = torch.rand(cnt * stride // rs, generator=g, device="cuda", dtype=dt) * 2 - 1
+        xout = torch.empty_like(xin)
+        b = mf.Batch([(kind, (n,), i * stride, i * stride) for i in range(cnt)],
+                     mf.PREC_FP32 if prec == 32 else mf.PREC_FP64)
+        ng = len(b.describe().strip().split("\n"))
+        ms = {}
+        for spec in ("rlong=0", "rlong=1"):
+            for gi in range(ng):
+                b.schedule(gi, spec)
+            ms[spec] = _time_enqueue(torch, lambda: b.run(xin.data_ptr(), xout.data_ptr(), sh), a.steps, a.warmup, st)
+        b.close()
+        # the first entry against the FP64 oracle
+        x0 = xin[: n].cpu().numpy().astype(np.float64)
+        ref = o.run(kind, x0, (n,))
+        odt = (torch.complex64 if prec == 32 else torch.complex128) if kind == "realdft" else dt
+        nout = n // 2 + 1 if kind == "realdft" else n
+        y0 = xout.view(torch.uint8)[: nout * (2 if kind == "realdft" else 1) * rs].view(odt).cpu().numpy()
+        err = float(np.linalg.norm(y0 - ref) / np.linalg.norm(ref))
+        alg = cnt * (ient + oent)
+        res[f"fp{prec}_{kind}"] = {
+            "entries": cnt, "two_pass_ms": round(ms["rlong=1"], 5), "four_pass_ms": round(ms["rlong=0"], 5),
+            "speedup": round(ms["rlong=0"] / ms["rlong=1"], 3),
+            "alg_hbm_gbs": round(alg / (ms["rlong=1"] * 1e-3) / 1e9, 1),
+            "frac": round(alg / (ms["rlong=1"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "traffic": read_traffic(f"kind={kind} prec={prec}", "rlong"), "alg_bytes": alg,
+            "rel_l2_vs_fp64_oracle": err}
+        del xin, xout
+        torch.cuda.empty_cache()
+    cpu = None
+    if not a.no_cpu and ref_available(32, fast=True):
+        R = Reference(32, fast=True)
+        ra = R.mkaux("realdft", (n,))
+        xh = np.random.default_rng(9).uniform(-1, 1, n).astype(np.float32)
+        zh = np.empty(n // 2 + 1, dtype=np.complex64)
+        t0 = time.perf_counter()
+        for _ in range(8):
+            R.fn("realdft")(xh.ctypes.data, zh.ctypes.data, ra)
+        dt = (time.perf_counter() - t0) / 8
+        R.free(ra)
+        cpu = {"value": round(2.5 * n * 20 / dt / 1e9, 3), "unit": "GFLOP/s (2.5 N log2 N)", "cores": 1,
+               "kind": "reference", "sample": f"8 FP32 realdft of 2^20 points, {dt*1e3:.2f} ms each"}
+    main = res["fp32_realdft"]
+    flops = 2.5 * n * 20 * main["entries"]
+    return _line("long 1D real DFT 2^20 FP32 GFLOP/s", round(flops / (main["two_pass_ms"] * 1e-3) / 1e9, 3),
+                 "GFLOP/s", main["two_pass_ms"], a.steps, a.warmup, "f32",
+                 "32 x 1D realdft of 2^20 points (and DCT-2/3/4; FP64 realdft), two HBM passes vs four",
+                 {"per_kind": res,
+                  "roofline": {"bound": "hbm", "achieved": main["alg_hbm_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                               "frac": main["frac"], "traffic": main["traffic"],
+                               "kernel": "rstep1_kernel + rstep2_kernel (2 passes)"},
+                  "cpu_baseline": cpu})
+
+
 def config_c4(a, torch, mf):
     """3D C2C 512^3 complex64 (FP32 path) vs the FP64 path on the same input."""
     n = 512
@@ -590,7 +663,7 @@ def main():
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison leg")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a 512 MiB device copy (calibrates the HBM byte counters)")
-    ap.add_argument("--config", default="c3", choices=["c1", "c2", "c3", "c4", "c5", "copy"],
+    ap.add_argument("--config", default="c3", choices=["c1", "c2", "c3", "c4", "c5", "rlong", "copy"],
                     help="c3 (default) is the metric's workload; the others print their own line")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -602,7 +675,8 @@ def main():
     import minfft_amd as mf
 
     if a.config != "c3":
-        fn = {"c1": config_c1, "c2": config_c2, "c4": config_c4, "c5": config_c5, "copy": config_copy}[a.config]
+        fn = {"c1": config_c1, "c2": config_c2, "c4": config_c4, "c5": config_c5, "rlong": config_rlong,
+              "copy": config_copy}[a.config]
         print(json.dumps(fn(a, torch, mf)), flush=True)
         return
 

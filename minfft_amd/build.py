@@ -46,7 +46,7 @@ CLANGXX = os.environ.get("MINFFT_CLANGXX", "/opt/rocm/llvm/bin/clang++")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 
 HEADERS = ["fft_core.hpp", "kinds.hpp", "tile.hpp", "ktable.hpp", "engine.hpp", "big.hpp", "bigtable.hpp", "mfma.hpp",
-           "autotune.hpp", "split2.hpp", "rsplit.hpp", "multi.hpp", "coop.hpp"]
+           "autotune.hpp", "split2.hpp", "rsplit.hpp", "multi.hpp", "coop.hpp", "rlong.hpp"]
 
 
 def _newer(target, deps):

@@ -14,6 +14,7 @@
 // in force (tune(): defaults, MINFFT_TUNE, or the autotuner's choice for the
 // plan / batch group).
 #include "engine.hpp"
+#include "rlong.hpp"
 
 #include <algorithm>
 #include <chrono>
@@ -222,6 +223,7 @@ bool parse_tune(const char* spec, Tune& t) {
         else if (k == "coop") t.coop = v;
         else if (k == "coopla" && v >= 1 && v <= 64) t.coop_la = v;
         else if (k == "mall" && v >= 0) t.mall_mb = v;
+        else if (k == "rlong") t.rlong = v;
         else ok = false;
     }
     return ok;
@@ -837,6 +839,79 @@ static int c2c_axis(const Workspace& ws, Prec prec, int kind, int logL, const vo
     return 0;
 }
 
+// A real/DCT/DST axis whose half-length FFT (L = 2^logL) is longer than a
+// tile, in two passes (rlong.hpp): L = N1*N2 with log2 N1 = logL/2; step 1
+// reads the input through the pass's levels (or entry table) into the
+// scratch ws.gbuf (transform b at b*L), step 2 writes the output through
+// them.  Returns 1 without launching for lengths outside the instantiated
+// step kernels.
+static int rlong_axis(const Workspace& ws, Prec prec, int kind, int logL, const void* in, long long in_e,
+                      void* out, long long out_e, const Levels& lv, hipStream_t st) {
+    const int l1 = logL / 2, l2 = logL - l1;
+    if (l1 < kRlongLogMin || l2 < kRlongLogMin || l1 > kRlongLogMax || l2 > kRlongLogMax) return 1;
+    const KTable& kt = ktable(prec);
+    const void* f1 = kt.rstep1[kind][l1];
+    const void* f2 = kt.rstep2[kind][l2];
+    if (!f1 || !f2) return 1;
+    const Device& D = device();
+    const size_t cs = prec == PREC_F64 ? 16 : 8;
+    const unsigned long long ntr = lv.count();
+    // columns (step 1) / rows (step 2) per work-group: up to 512 lanes, LDS
+    // within the device limit, at least two rows (one pair) in step 2
+    auto per_tile = [&](int lfft, int lother, int cmin) {
+        const int tpt = (1 << lfft) / default_E(lfft);
+        const size_t slot = (size_t)slot_len(1 << lfft) * cs;
+        int C = std::min(1 << lother, std::max(1, 512 / tpt));
+        while (C > cmin && (size_t)C * slot > (size_t)D.lds_max) C >>= 1;
+        return (C >= cmin && (size_t)C * slot <= (size_t)D.lds_max) ? C : 0;
+    };
+    const int C1 = per_tile(l1, l2, 1), C2 = per_tile(l2, l1, 2);
+    if (!C1 || !C2) return 1;
+    const unsigned long long t1 = ntr << (l2 - ilog2i(C1)), t2 = ntr << (l1 - ilog2i(C2));
+    if (t1 > 0xffffffffull || t2 > 0xffffffffull) return 1;
+
+    TileArgs A;
+    std::memset(&A, 0, sizeof(A));
+    fill_levels(A, lv);
+    A.ntrans = (unsigned)ntr;
+    A.tw = ws.tw;
+    A.tw64 = (const double2*)ws.tw64;
+    A.fs_logN = logL;
+    A.in = in;
+    A.in_e = in_e;
+    A.out = ws.gbuf;
+    A.out_off = nullptr;
+    A.C = C1;
+    A.lgC = ilog2i(C1);
+    A.sc = slot_len(1 << l1);
+    A.ntiles = (unsigned)t1;
+    A.xcd = t1 >= 2ull * D.nxcd ? D.nxcd : 1;
+    const unsigned b1 = (unsigned)(C1 * ((1 << l1) / default_E(l1)));
+    void* a1[] = {&A};
+    int rc = launch_fn(f1, (unsigned)t1, b1, a1, (size_t)C1 * A.sc * cs, st);
+    if (rc) return rc;
+
+    TileArgs B;
+    std::memset(&B, 0, sizeof(B));
+    fill_levels(B, lv);
+    B.ntrans = (unsigned)ntr;
+    B.tw = ws.tw;
+    B.tw64 = (const double2*)ws.tw64;
+    B.fs_logN = logL;
+    B.in = ws.gbuf;
+    B.in_off = nullptr;
+    B.out = out;
+    B.out_e = out_e;
+    B.C = C2;
+    B.lgC = ilog2i(C2);
+    B.sc = slot_len(1 << l2);
+    B.ntiles = (unsigned)t2;
+    B.xcd = t2 >= 2ull * D.nxcd ? D.nxcd : 1;
+    const unsigned b2 = (unsigned)(C2 * ((1 << l2) / default_E(l2)));
+    void* a2[] = {&B};
+    return launch_fn(f2, (unsigned)t2, b2, a2, (size_t)C2 * B.sc * cs, st);
+}
+
 int axis_pass(const Workspace& ws, Prec prec, int kind, int logN, const void* in, long long in_e,
               void* out, long long out_e, const Levels& lv, hipStream_t st) {
     if (!ensure_device()) return -1;
@@ -860,8 +935,16 @@ int axis_pass(const Workspace& ws, Prec prec, int kind, int logN, const void* in
     const int logL = logN - 1;
     if (logL <= maxlog)
         return launch_tile(prec, kind, logL, ws.tw, ws.tw64, in, in_e, out, out_e, lv, -1, 0, st);
+    if (!ws.gbuf) {
+        set_error("internal: missing two-step scratch");
+        return -1;
+    }
+    if (tune().rlong) {
+        const int rc = rlong_axis(ws, prec, kind, logL, in, in_e, out, out_e, lv, st);
+        if (rc <= 0) return rc;  // 1: lengths outside the two-pass kernels
+    }
     // long real transform: pre-kernel -> two-step complex FFT -> post-kernel
-    if (!ws.gbuf || !ws.fbuf) {
+    if (!ws.fbuf) {
         set_error("internal: missing two-step scratch");
         return -1;
     }

@@ -1,8 +1,9 @@
-// inst.hip -- instantiates the tile, pre, post and length-1 kernels of one
-// (precision, kind) pair.  Compiled once per pair with
+// inst.hip -- instantiates the tile, pre, post, two-pass long-transform and
+// length-1 kernels of one (precision, kind) pair.  Compiled once per pair with
 //   -DINST_F64=0|1 -DINST_KIND=<0..9>
 // so the 20 units build in parallel.
 #include "tile.hpp"
+#include "rlong.hpp"
 #include "ktable.hpp"
 
 #ifndef INST_F64
@@ -64,7 +65,20 @@ static void reg_tiles32(KTable& t) {
 #endif
 }
 
+// two-pass step kernels of the long real/DCT/DST transforms
+template <int LOG>
+static void reg_rlong(KTable& t) {
+    if constexpr (INST_KIND != K_DFT && INST_KIND != K_INVDFT && LOG <= kRlongLogMax) {
+        t.rstep1[INST_KIND][LOG] = reinterpret_cast<const void*>(&rstep1_kernel<IT, LOG, INST_KIND>);
+        t.rstep2[INST_KIND][LOG] = reinterpret_cast<const void*>(&rstep2_kernel<IT, LOG, INST_KIND>);
+        reg_rlong<LOG + 1>(t);
+    } else {
+        (void)t;
+    }
+}
+
 void FNAME(KTable& t) {
+    reg_rlong<kRlongLogMin>(t);
     reg_tiles<0>(t);
     reg_tiles32<9>(t);
     t.maxlog = IMAX;

@@ -49,9 +49,11 @@ template <typename T> __device__ __forceinline__ T sqrt2_v() { return (T)1.41421
 // RI(i) returns real input item i; CI(i) complex input item i.
 // ---------------------------------------------------------------------------
 
-// inverse real DFT pre-twiddle, reading z[0..L] through Z(i)
-template <typename T, class ZF>
-__device__ __forceinline__ c2<T> c2r_pre(int p, int L, const c2<T>* __restrict__ tw, ZF Z) {
+// inverse real DFT pre-twiddle, reading z[0..L] through Z(i).  The twiddle
+// source tw is indexed as the table (tw[M + m] = W_M^m): a pointer to it or
+// an accessor with the same operator[].
+template <typename T, class TWT, class ZF>
+__device__ __forceinline__ c2<T> c2r_pre(int p, int L, TWT tw, ZF Z) {
     if (p == 0) {
         // imaginary parts of z[0] and z[L] are ignored (minfft.c:629-630)
         T a = Z(0).x, b = Z(L).x;
@@ -76,8 +78,8 @@ __device__ __forceinline__ c2<T> c2r_pre(int p, int L, const c2<T>* __restrict__
 }
 
 // real DFT post-twiddle: output item k (0 <= k <= L) from FFT output T(i)
-template <typename T, class TF>
-__device__ __forceinline__ c2<T> r2c_post(int k, int L, const c2<T>* __restrict__ tw, TF Tf) {
+template <typename T, class TWT, class TF>
+__device__ __forceinline__ c2<T> r2c_post(int k, int L, TWT tw, TF Tf) {
     if (k == 0) {
         c2<T> u = Tf(0);
         return mk<T>(u.x + u.y, (T)0);
@@ -110,8 +112,8 @@ __device__ __forceinline__ T t2_reorder(int i, int L, RF X) {
 }
 
 // DCT-3/DST-3 intermediate z[k] (0 <= k <= L) from the real input
-template <typename T, bool DST, class RF>
-__device__ __forceinline__ c2<T> t3_z(int k, int L, const c2<T>* __restrict__ tw, RF X) {
+template <typename T, bool DST, class TWT, class RF>
+__device__ __forceinline__ c2<T> t3_z(int k, int L, TWT tw, RF X) {
     const int N = 2 * L;
     if (k == 0) return mk<T>(DST ? X(N - 1) : X(0), (T)0);
     if (k == L) return mk<T>(sqrt2_v<T>() * (DST ? X(L - 1) : X(L)), (T)0);
@@ -122,8 +124,8 @@ __device__ __forceinline__ c2<T> t3_z(int k, int L, const c2<T>* __restrict__ tw
 }
 
 // FFT input point p for every kind
-template <typename T, int KIND, class RF, class CF>
-__device__ __forceinline__ c2<T> pre_point(int p, int L, const c2<T>* __restrict__ tw, RF X, CF Z) {
+template <typename T, int KIND, class TWT, class RF, class CF>
+__device__ __forceinline__ c2<T> pre_point(int p, int L, TWT tw, RF X, CF Z) {
     if constexpr (KIND == K_DFT || KIND == K_INVDFT || KIND == K_REALDFT) {
         return Z(p);
     } else if constexpr (KIND == K_INVREALDFT) {
@@ -146,12 +148,32 @@ __device__ __forceinline__ c2<T> pre_point(int p, int L, const c2<T>* __restrict
     }
 }
 
+// DCT-2/DST-2: both output items anchored at FFT position n (0 <= n < L)
+// from one real-DFT value Zf(n) and one twiddle -- the formulas of
+// post_item's DCT-2/DST-2 branch for items n and N - n (DCT-2) or n - 1 and
+// N - 1 - n (DST-2); for n = 0 the items 0 and L (DCT-2) or N - 1 and L - 1
+// (DST-2), which use Zf(0) and Zf(L).
+template <typename T, bool DST, class TWT, class TF>
+__device__ __forceinline__ void t2_post_pair(int n, int L, TWT tw, TF Tf, T& a, T& b) {
+    const int N = 2 * L;
+    if (n == 0) {
+        a = 2 * r2c_post<T>(0, L, tw, Tf).x;
+        b = sqrt2_v<T>() * r2c_post<T>(L, L, tw, Tf).x;
+        return;
+    }
+    const c2<T> z = r2c_post<T>(n, L, tw, Tf);
+    const c2<T> e = tw[4 * N + n];
+    const T re = 2 * (z.x * e.x - z.y * e.y), im = -2 * (z.x * e.y + z.y * e.x);
+    a = DST ? im : re;  // DCT-2 item n (re) / DST-2 item n - 1 (re = false)
+    b = DST ? re : im;  // DCT-2 item N - n / DST-2 item N - 1 - n
+}
+
 // ---------------------------------------------------------------------------
 // post-processing: output item i from the FFT output T(p), p < L.
 // Complex kinds return the complex item; real kinds return .x only.
 // ---------------------------------------------------------------------------
-template <typename T, int KIND, class TF>
-__device__ __forceinline__ c2<T> post_item(int i, int L, const c2<T>* __restrict__ tw, TF Tf) {
+template <typename T, int KIND, class TWT, class TF>
+__device__ __forceinline__ c2<T> post_item(int i, int L, TWT tw, TF Tf) {
     if constexpr (KIND == K_DFT || KIND == K_INVDFT) {
         return Tf(i);
     } else if constexpr (KIND == K_REALDFT) {

@@ -20,6 +20,10 @@ struct KTable {
     // small-pass complex tiles, 4 points per lane ([inverse][logL], logL
     // kSmallLogMin..kSmallLogMax; null elsewhere)
     const void* tile4[2][kMaxLog + 1];
+    // long real/DCT/DST transforms in two passes (rlong.hpp): step kernels by
+    // log2 N1 / log2 N2 (kRlongLogMin..kRlongLogMax; null elsewhere)
+    const void* rstep1[K_COUNT][kMaxLog + 1];
+    const void* rstep2[K_COUNT][kMaxLog + 1];
     int maxlog;                               // largest logL instantiated
     const void* pre[K_COUNT];
     const void* post[K_COUNT];
