@@ -74,6 +74,14 @@ void register_big(BigTable& t) {
     t.mfma512nt[1] = reinterpret_cast<const void*>(&mfma512_kernel<true, true>);
     t.mfma512cfnt[0] = reinterpret_cast<const void*>(&mfma512_cf_kernel<false, true>);
     t.mfma512cfnt[1] = reinterpret_cast<const void*>(&mfma512_cf_kernel<true, true>);
+    t.mfma2[0][0][0] = reinterpret_cast<const void*>(&mfma512_kernel<false, false, 2>);
+    t.mfma2[0][0][1] = reinterpret_cast<const void*>(&mfma512_kernel<true, false, 2>);
+    t.mfma2[0][1][0] = reinterpret_cast<const void*>(&mfma512_kernel<false, true, 2>);
+    t.mfma2[0][1][1] = reinterpret_cast<const void*>(&mfma512_kernel<true, true, 2>);
+    t.mfma2[1][0][0] = reinterpret_cast<const void*>(&mfma512_cf_kernel<false, false, 2>);
+    t.mfma2[1][0][1] = reinterpret_cast<const void*>(&mfma512_cf_kernel<true, false, 2>);
+    t.mfma2[1][1][0] = reinterpret_cast<const void*>(&mfma512_cf_kernel<false, true, 2>);
+    t.mfma2[1][1][1] = reinterpret_cast<const void*>(&mfma512_cf_kernel<true, true, 2>);
     // <T, log2 H, min waves per SIMD>: the first length past one
     // work-group's LDS, and the two lengths below it (two work-groups per CU)
     reg_split2<float, 14, 4>(t, 1);

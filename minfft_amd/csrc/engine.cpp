@@ -224,6 +224,7 @@ bool parse_tune(const char* spec, Tune& t) {
         else if (k == "coopla" && v >= 1 && v <= 64) t.coop_la = v;
         else if (k == "mall" && v >= 0) t.mall_mb = v;
         else if (k == "rlong") t.rlong = v;
+        else if (k == "mstages" && (v == 1 || v == 2)) t.mstages = v;
         else ok = false;
     }
     return ok;
@@ -481,9 +482,17 @@ static int launch_tile(Prec prec, int kind, int logL, const void* tw, const void
     // I/O keeps its work-group barriers (tile.hpp)
     A.wave = (tn.wave && !A.cfast) ? 1 : 0;
     const BigTable& btab = bigtable();
-    const bool plain_tile = fn != btab.mfma512[0] && fn != btab.mfma512[1] && fn != btab.mfma512cf[0] &&
-                            fn != btab.mfma512cf[1] && fn != btab.mfma512nt[0] && fn != btab.mfma512nt[1] &&
-                            fn != btab.mfma512cfnt[0] && fn != btab.mfma512cfnt[1];
+    // both radix-16 stages on the matrix cores (schedule key mstages=2)
+    if (split16 && tn.mstages == 2) {
+        const int iv = kind == K_INVDFT;
+        for (int cf = 0; cf < 2; ++cf)
+            for (int nt = 0; nt < 2; ++nt) {
+                const void* one = cf ? (nt ? btab.mfma512cfnt[iv] : btab.mfma512cf[iv])
+                                     : (nt ? btab.mfma512nt[iv] : btab.mfma512[iv]);
+                if (fn == one) fn = btab.mfma2[cf][nt][iv];
+            }
+    }
+    const bool plain_tile = !split16;
     if (t_signal && !t_signal->used && tiles == 1 && plain_tile) {
         A.hflag = t_signal->flag;
         A.hval = t_signal->value;

@@ -102,6 +102,8 @@ struct Tune {
     int mall_mb = 0;    // multi-dimensional passes on chunks of entries / slabs of
                         // about this many MiB, so the later passes read the MALL (0 = off:
                         // measured slower at every size, see run_entries)
+    int mstages = 1;    // split-FP16 512-point tiles: radix-16 stages on the matrix cores
+                        // (1: the first; 2: both -- measured slower, mfma.hpp)
     int rlong = 1;      // real/DCT/DST axes longer than a tile: 1 = two passes with the
                         // pre- and post-processing fused into the two-step FFT (rlong.hpp),
                         // 0 = pre-kernel, two-step FFT, post-kernel

@@ -3,7 +3,7 @@
 // batch extension, MINFFT_SPLIT_FP16).
 //
 // L = 512 = 16 * 16 * 2, Stockham passes (Ns = 1, 16, 256):
-//   pass 1, radix 16 on MFMA: for a group of 16 columns (butterflies, or
+//   pass 1, radix 16 on MFMA (no twiddles): for a group of 16 columns (butterflies, or
 //     transforms) the 16 inputs x_r (rows r) form a 16x16 complex block;
 //     X = F16 * x with F16[k][r] = exp(-+2 pi i k r / 16), in real arithmetic
 //         Xr = C*xr + S*xi,  Xi = C*xi - S*xr     (forward; S -> -S inverse)
@@ -17,8 +17,11 @@
 //     scaled by a power of two first, so that the largest component lies in
 //     [2^13, 2^14): FP16 never overflows and the error stays relative to the
 //     data's own magnitude; the scale is undone exactly on the FP32 result.
-//   passes 2 and 3 (radix 16 with twiddles W_256, radix 2 with W_512) run in
-//     FP32 on the vector ALUs, as in fft_core.hpp.
+//   pass 2, radix 16 with twiddles W_256^{(j % 16) r}: FP32 on the vector
+//     ALUs, as in fft_core.hpp; or (STAGES = 2, schedule key mstages=2) the
+//     same product on the matrix cores, its inputs twiddled in FP32 first
+//     and scaled anew;
+//   pass 3, radix 2 with W_512, FP32 on the vector ALUs.
 // Operand layouts of v_mfma_f32_16x16x32_f16 (lane l, element j = 0..7,
 // q = 0..3): A[l%16][8*(l/16)+j], B[8*(l/16)+j][l%16], D[4*(l/16)+q][l%16].
 #pragma once
@@ -153,6 +156,34 @@ __device__ __forceinline__ float4_t mfma32_product(half8_t ah, half8_t al, half8
     return d;
 }
 
+// Power-of-two exponent that brings m (a largest |component|) into
+// [2^13, 2^14): FP16 never overflows and the split error stays relative to
+// the data's own magnitude
+__device__ __forceinline__ int scale_exp(float m) {
+    int ex = 0;
+    (void)frexpf(m, &ex);
+    return (m > 0.f) ? 14 - ex : 0;
+}
+
+// One 16-column group of a radix-16 stage: lane (c, g) holds inputs
+// r = 4g + q (q < 4) of column c in x; scaled by 2^sh, DFT-16 on the matrix
+// cores, unscaled; output k = 4g + q of column c returned in y[q]
+__device__ __forceinline__ void mfma16_columns(const float2 (&x)[4], int sh, const Mfma32Operands& op, float m1,
+                                               float2 (&y)[4]) {
+    float xr[4], xi[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        xr[q] = ldexpf(x[q].x, sh);
+        xi[q] = ldexpf(x[q].y, sh);
+    }
+    half8_t bh, bl;
+    mfma32_operand_b(xr, xi, m1, bh, bl);
+    const float4_t Xr = mfma32_product(op.arh, op.arl, bh, bl);
+    const float4_t Xi = mfma32_product(op.aih, op.ail, bh, bl);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) y[q] = make_float2(ldexpf(Xr[q], -sh), ldexpf(Xi[q], -sh));
+}
+
 // C = A.C transforms per work-group (even, at most 16), 32 lanes each; every
 // wavefront owns two whole transforms (2w, 2w+1), so apart from staged
 // column I/O the work-group's wavefronts never wait on each other.
@@ -161,7 +192,7 @@ __device__ __forceinline__ float4_t mfma32_product(half8_t ah, half8_t al, half8
 // x[2c + 1 + 32r] (r = 4g + q) as one 16-byte load straight from memory for
 // contiguous rows; the LDS exchanges after pass 1 stay inside the wavefront.
 // NT: direct I/O with non-temporal accesses (passes larger than the MALL)
-template <bool INV, bool NT = false>
+template <bool INV, bool NT = false, int STAGES = 1>
 __global__ void __launch_bounds__(512) mfma512_kernel(TileArgs A) {
     using C2 = float2;
     constexpr int L = 512, E = 16, TPT = L / E;
@@ -282,14 +313,65 @@ __global__ void __launch_bounds__(512) mfma512_kernel(TileArgs A) {
     }
     tile_sync(true);
 
-    // ---- passes 2 and 3 on the vector ALUs, inside the wavefront ----
     C2 v[E];
-    Core<float, 9, E, INV>::read_owned(v, slot, tl);
-    tile_sync(true);
-    stockham_pass<float, L, E, 16, 16, INV, false>(v, slot, tl, tw);
-    tile_sync(true);
-    Core<float, 9, E, INV>::read_owned(v, slot, tl);
-    stockham_pass<float, L, E, 2, 256, INV, true>(v, slot, tl, tw);
+    if constexpr (STAGES == 2) {
+        // ---- pass 2 (radix 16, twiddles W_256^{(j % 16) r}) on the matrix
+        // cores: inputs x[j + 32r], r = 4g + q, of butterflies j = 2cc + h from
+        // the wavefront's slots, twiddled in FP32, one scale per transform;
+        // output k goes to 256 (j / 16) + j % 16 + 16k ----
+        {
+            float2 x2[2][2][4];
+#pragma unroll
+            for (int tt = 0; tt < 2; ++tt) {
+                const C2* s = wslot + (long long)tt * A.sc;
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    const int j = 2 * cc + h;
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        const int r = 4 * g + q;
+                        const C2 a = s[j + (j >> 4) + 34 * r];  // lpad(j + 32r)
+                        const C2 wt = tw[256 + (j & 15) * r];
+                        x2[tt][h][q] = INV ? cmulc<float>(a, wt) : cmul<float>(a, wt);
+                    }
+                }
+            }
+            tile_sync(true);  // every input read before the slots are overwritten
+#pragma unroll
+            for (int tt = 0; tt < 2; ++tt) {
+                float m = 0.f;
+#pragma unroll
+                for (int h = 0; h < 2; ++h)
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) m = fmaxf(m, fmaxf(fabsf(x2[tt][h][q].x), fabsf(x2[tt][h][q].y)));
+                const int sh2 = __builtin_amdgcn_readfirstlane(scale_exp(wave_max(m)));
+                C2* s = wslot + (long long)tt * A.sc;
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    const int j = 2 * cc + h;
+                    float2 y[4];
+                    mfma16_columns(x2[tt][h], sh2, op, m1, y);
+                    // lpad(256 (j >> 4) + (j & 15) + 16k) = 272 (j >> 4) + (j & 15) + 17k
+                    C2* d = s + (272 * (j >> 4) + (j & 15));
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) d[17 * (4 * g + q)] = y[q];
+                }
+            }
+        }
+        tile_sync(true);
+
+        // ---- pass 3 (radix 2, W_512) on the vector ALUs, inside the wavefront ----
+        Core<float, 9, E, INV>::read_owned(v, slot, tl);
+        stockham_pass<float, L, E, 2, 256, INV, true>(v, slot, tl, tw);
+    } else {
+        // ---- passes 2 and 3 on the vector ALUs, inside the wavefront ----
+        Core<float, 9, E, INV>::read_owned(v, slot, tl);
+        tile_sync(true);
+        stockham_pass<float, L, E, 16, 16, INV, false>(v, slot, tl, tw);
+        tile_sync(true);
+        Core<float, 9, E, INV>::read_owned(v, slot, tl);
+        stockham_pass<float, L, E, 2, 256, INV, true>(v, slot, tl, tw);
+    }
 
     // ---- output ----
     if (A.out_mode == IO_DIRECT) {
@@ -327,7 +409,7 @@ __global__ void __launch_bounds__(512) mfma512_kernel(TileArgs A) {
 // and 3 run
 // with the c-fast core mapping (lane t: transform t % 16, thread t / 16),
 // whose owned positions give coalesced direct stores.
-template <bool INV, bool NT = false>
+template <bool INV, bool NT = false, int STAGES = 1>
 __global__ void __launch_bounds__(512) mfma512_cf_kernel(TileArgs A) {
     using C2 = float2;
     constexpr int L = 512, E = 16, TPT = L / E, CT = 16;
@@ -367,48 +449,76 @@ __global__ void __launch_bounds__(512) mfma512_cf_kernel(TileArgs A) {
     // ---- pass 1 on the matrix cores: output k = 4g + q of butterfly j of
     // transform c goes to position 16j + k of slot c.  One scale for the
     // wavefront's four columns of transform c (lanes c + 16g): their largest
-    // |component| -> [2^13, 2^14) ----
-    float m = 0.f;
+    // |component| ----
+    auto group_scale = [&]() {
+        float m = 0.f;
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+        for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int q = 0; q < 4; ++q) m = fmaxf(m, fmaxf(fabsf(x[i][q].x), fabsf(x[i][q].y)));
-    m = fmaxf(m, __shfl_xor(m, 16));
-    m = fmaxf(m, __shfl_xor(m, 32));
-    int ex = 0;
-    (void)frexpf(m, &ex);
-    const int sh = (m > 0.f) ? 14 - ex : 0;
+            for (int q = 0; q < 4; ++q) m = fmaxf(m, fmaxf(fabsf(x[i][q].x), fabsf(x[i][q].y)));
+        m = fmaxf(m, __shfl_xor(m, 16));
+        m = fmaxf(m, __shfl_xor(m, 32));
+        return scale_exp(m);
+    };
+    {
+        const int sh = group_scale();
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        const int j = 4 * w + i;
-        // the 32-row operand [x_r ; x_i]: lanes of the lower half keep their
-        // real parts and take those of lane + 32; lanes of the upper half
-        // keep their imaginary parts and take those of lane - 32
-        float xr[4], xi[4];
+        for (int i = 0; i < 4; ++i) {
+            const int j = 4 * w + i;
+            float2 y[4];
+            mfma16_columns(x[i], sh, op, m1, y);
+            // position 16j + 4g + q: padded 17j + 4g + q
+            C2* d = slot + (17 * j + 4 * g);
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            xr[q] = ldexpf(x[i][q].x, sh);
-            xi[q] = ldexpf(x[i][q].y, sh);
+            for (int q = 0; q < 4; ++q) d[q] = y[q];
         }
-        half8_t bh, bl;
-        mfma32_operand_b(xr, xi, m1, bh, bl);
-        const float4_t Xr = mfma32_product(op.arh, op.arl, bh, bl);
-        const float4_t Xi = mfma32_product(op.aih, op.ail, bh, bl);
-        // position 16j + 4g + q: padded 17j + 4g + q
-        C2* d = slot + (17 * j + 4 * g);
-#pragma unroll
-        for (int q = 0; q < 4; ++q) d[q] = make_float2(ldexpf(Xr[q], -sh), ldexpf(Xi[q], -sh));
     }
     __syncthreads();
 
-    // ---- passes 2 and 3 on the vector ALUs, c-fast core ----
     C2 v[E];
-    Core<float, 9, E, INV>::read_owned(v, slot, tl);
-    __syncthreads();
-    stockham_pass<float, L, E, 16, 16, INV, false>(v, slot, tl, tw);
-    __syncthreads();
-    Core<float, 9, E, INV>::read_owned(v, slot, tl);
-    stockham_pass<float, L, E, 2, 256, INV, true>(v, slot, tl, tw);
+    if constexpr (STAGES == 2) {
+        // ---- pass 2 (radix 16, twiddles W_256^{(j % 16) r}) on the matrix
+        // cores: inputs x_c[j + 32r], r = 4g + q, of butterflies j = 4w + i from
+        // the slot, twiddled in FP32; output k goes to 256 (j / 16) + j % 16 + 16k ----
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int j = 4 * w + i;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int r = 4 * g + q;
+                const C2 a = slot[j + (j >> 4) + 34 * r];  // lpad(j + 32r)
+                const C2 wt = tw[256 + (j & 15) * r];
+                x[i][q] = INV ? cmulc<float>(a, wt) : cmul<float>(a, wt);
+            }
+        }
+        __syncthreads();  // every input read before the slot is overwritten
+        {
+            const int sh = group_scale();
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int j = 4 * w + i;
+                float2 y[4];
+                mfma16_columns(x[i], sh, op, m1, y);
+                // lpad(256 (j >> 4) + (j & 15) + 16k) = 272 (j >> 4) + (j & 15) + 17k
+                C2* d = slot + (272 * (j >> 4) + (j & 15));
+#pragma unroll
+                for (int q = 0; q < 4; ++q) d[17 * (4 * g + q)] = y[q];
+            }
+        }
+        __syncthreads();
+
+        // ---- pass 3 (radix 2, W_512) on the vector ALUs, c-fast core ----
+        Core<float, 9, E, INV>::read_owned(v, slot, tl);
+        stockham_pass<float, L, E, 2, 256, INV, true>(v, slot, tl, tw);
+    } else {
+        // ---- passes 2 and 3 on the vector ALUs, c-fast core ----
+        Core<float, 9, E, INV>::read_owned(v, slot, tl);
+        __syncthreads();
+        stockham_pass<float, L, E, 16, 16, INV, false>(v, slot, tl, tw);
+        __syncthreads();
+        Core<float, 9, E, INV>::read_owned(v, slot, tl);
+        stockham_pass<float, L, E, 2, 256, INV, true>(v, slot, tl, tw);
+    }
 
     // ---- output: neighbouring lanes store neighbouring transforms ----
     if (me.valid) {

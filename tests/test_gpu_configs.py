@@ -174,8 +174,10 @@ def _plane_waves(torch, n, seed=4, count=64):
     return field, ks, phis
 
 
-def _split16(torch, x, y):
+def _split16(torch, x, y, spec=None):
     plan = mf.Batch([("dft", tuple(x.shape), 0, 0)], mf.PREC_SPLIT_FP16)
+    if spec:
+        plan.schedule(0, spec)
     plan.run(x.data_ptr(), y.data_ptr(), None)
     torch.cuda.synchronize()
     desc = plan.describe()
@@ -207,6 +209,11 @@ def test_512_cubed_split_fp16_plane_waves(lib32):
     e16 = err_vs_closed_form(y)
     print(f"  split-FP16 rel-L2 vs closed form {e16:.3e}", flush=True)
     assert e16 <= TOL32, e16
+    # both radix-16 stages on the matrix cores (schedule key mstages=2)
+    _split16(torch, x, y, "mstages=2")
+    e16b = err_vs_closed_form(y)
+    print(f"  split-FP16, both radix-16 stages, rel-L2 vs closed form {e16b:.3e}", flush=True)
+    assert e16b <= TOL32, e16b
     # the FP32 form of the plain API on the same input
     aux = mf.Aux(lib32, "dft", (n, n, n))
     aux.enqueue("dft", x, y)

@@ -212,7 +212,8 @@ def test_autotuner_single_group_plan(oracle64):
 def test_split_fp16_schedules(oracle64):
     """The split-FP16 MFMA tiles under the schedules that change their launch
     shape: column passes on the column variant (c-fast core) or on the row
-    variant with LDS-staged column input and output (cfcore=0)."""
+    variant with LDS-staged column input and output (cfcore=0); and with
+    both radix-16 stages on the matrix cores (mstages=2)."""
     import torch
     shapes = [(512, 64), (64, 512), (512, 8, 2), (8, 512, 4)]
     kinds = ["dft", "invdft", "dft", "invdft"]
@@ -229,7 +230,8 @@ def test_split_fp16_schedules(oracle64):
     dout = torch.zeros(pos, dtype=torch.uint8, device="cuda")
     batch = mf.Batch([(k, sh, o, o) for k, sh, o in zip(kinds, shapes, offs)], mf.PREC_SPLIT_FP16)
     refs = [oracle64.run(k, x.astype(np.complex128), sh) for k, x, sh in zip(kinds, xs, shapes)]
-    for spec in ("", "cfcore=0", "cfcore=1", "cfcore=0,cfb=256", "xcd=0"):
+    for spec in ("", "cfcore=0", "cfcore=1", "cfcore=0,cfb=256", "xcd=0", "mstages=2", "mstages=2,cfcore=0",
+                 "mstages=2,cfcore=1"):
         for g in range(len(shapes)):
             batch.schedule(g, spec)
         print(f"[case] split-FP16 schedule '{spec}'", flush=True)
