@@ -135,8 +135,8 @@ struct Plan {
 // the schedule in force on this thread
 const Tune& tune();
 // applies "key=value,..." (the Tune fields: tpw, cfb, wave, xcd, cfcore,
-// e32, stage, chunk, split, split2, fused, wpc, falt, rsplit, coop, coopla,
-// mall) to t; false on an unknown key
+// e32, e4, stage, chunk, split, split2, fused, wpc, falt, rsplit, coop,
+// coopla, mall, rlong, mstages) to t; false on an unknown key
 bool parse_tune(const char* spec, Tune& t);
 // the process-wide schedule (defaults + MINFFT_TUNE)
 const Tune& base_tune();
