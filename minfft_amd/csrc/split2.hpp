@@ -12,7 +12,8 @@
 // owned positions n = tl + m*TPT of both halves, forms a and b in
 // registers, runs DFT_H(a) through the LDS slot while b waits in registers,
 // then DFT_H(b) while DFT_H(a)'s result waits, and writes the pairs
-// (X[2k], X[2k+1]) of its owned k -- adjacent lanes, adjacent pairs.
+// (X[2k], X[2k+1]) of its owned k -- adjacent lanes, adjacent pairs, one
+// 16-byte store per pair in FP32.
 // NT: input and output move with non-temporal accesses -- for whole 1D
 // transforms (streamed once: in a batch they would otherwise displace the
 // fused kernel's scratch slots of the groups beside them) and passes larger
@@ -57,6 +58,26 @@ __global__ void __launch_bounds__((1 << LOGH) / E, WPE) split2_kernel(TileArgs A
     __syncthreads();                              // slot free for the second half
     Core<T, LOGH, E, INV>::run(w, slot, tl, tw);  // w: X[2k+1]
     C2* __restrict__ out = reinterpret_cast<C2*>(A.out) + me.ob;
+    // the pair (X[2k], X[2k+1]) as one 16-byte store when it is aligned
+    // (FP32, outputs at a 16-byte boundary -- uniform over the work-group)
+    if constexpr (sizeof(T) == 4) {
+        if ((reinterpret_cast<unsigned long long>(out) & 15) == 0) {
+            typedef float V4 __attribute__((ext_vector_type(4)));
+#pragma unroll
+            for (int m = 0; m < E; ++m) {
+                const int k = tl + m * TPT;
+                V4 q;
+                q.x = v[m].x;
+                q.y = v[m].y;
+                q.z = w[m].x;
+                q.w = w[m].y;
+                V4* p = reinterpret_cast<V4*>(out + 2 * k);
+                if constexpr (NT) __builtin_nontemporal_store(q, p);
+                else *p = q;
+            }
+            return;
+        }
+    }
 #pragma unroll
     for (int m = 0; m < E; ++m) {
         const int k = tl + m * TPT;
