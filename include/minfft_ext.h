@@ -51,8 +51,8 @@ typedef struct minfft_batch_plan minfft_batch_plan;
 
 /* Build a batch plan: validates entries (NULL on error), groups identical
    shapes, builds the twiddle tables on the device, gathers the short 1D
-   complex groups (2^4..2^12 points) of each direction into one persistent
-   launch (off with MINFFT_MULTI=0) and picks each other group's launch
+   complex groups (2^4..2^12 points) of each direction into one launch
+   (off with MINFFT_MULTI=0) and picks each other group's launch
    schedule by timing candidates on the device (autotuner; off with
    MINFFT_AUTOTUNE=0; results cached per shape, and in the file named by
    MINFFT_TUNE_CACHE if set). */

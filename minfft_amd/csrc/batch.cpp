@@ -98,6 +98,21 @@ static bool multi_enabled() {
     return !(e && std::atoi(e) == 0);
 }
 
+// Form of the short-length launch.  Alone (the plan's only unit of work)
+// one work-group per item is faster -- no counter, no reset launch, the
+// hardware spreads the work-groups (5220 entries of 2^6..2^12, FP32:
+// 21.8-22.1 against 36.5-36.6 us); beside other groups the persistent
+// form is, because its ~1000 resident work-groups leave CUs to the long
+// groups on the critical path (the default 8192-entry step: 409.4 against
+// 413.5-415.6 us).  MINFFT_MULTI=1 / 2 force the persistent / per-item form.
+static bool multi_per_item(size_t units) {
+    const char* e = std::getenv("MINFFT_MULTI");
+    const int v = e ? std::atoi(e) : -1;
+    if (v == 1) return false;
+    if (v == 2) return true;
+    return units <= 1;
+}
+
 // groups run on up to this many concurrent graph branches (default 8: in
 // practice one branch per unit of work; the default batch's five units ran
 // 0.409 ms per step on five branches against 0.417 ms on four)
@@ -412,7 +427,7 @@ static int enqueue_batch(minfft_batch_plan* p, const void* in, void* out, hipStr
             MultiLaunch& ml = p->multi[-1 - u];
             ml.args.in = in;
             ml.args.out = out;
-            return launch_multi(p->prec, ml.inv, ml.args, ml.grid, s);
+            return launch_multi(p->prec, ml.inv, ml.args, ml.grid, multi_per_item(units.size()), s);
         }
         BatchGroup& g = p->groups[u];
         TuneScope scope(&g.tune.sched);

@@ -631,13 +631,19 @@ int multi_grid(Prec prec) {
     return occ * device().ncu;
 }
 
-int launch_multi(Prec prec, bool inv, const MultiArgs& a, unsigned grid, hipStream_t st) {
+int launch_multi(Prec prec, bool inv, const MultiArgs& a, unsigned grid, bool per_item, hipStream_t st) {
     const BigTable& bt = bigtable();
     const int pi = prec == PREC_F64 ? 0 : 1;
+    MultiArgs m = a;
+    if (per_item) {
+        // one work-group per item: the grid is the item count
+        m.counter = nullptr;
+        void* args[] = {&m};
+        return launch_fn(bt.multi[pi][inv ? 1 : 0], a.total, kMultiThreads, args, bt.multi_lds[pi], st);
+    }
     unsigned* ctr = a.counter;
     void* rargs[] = {&ctr};
     if (launch_fn(bt.multi_reset, 1, 1, rargs, 0, st)) return -1;
-    MultiArgs m = a;
     void* args[] = {&m};
     return launch_fn(bt.multi[pi][inv ? 1 : 0], grid, kMultiThreads, args, bt.multi_lds[pi], st);
 }

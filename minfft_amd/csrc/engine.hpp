@@ -200,7 +200,10 @@ int multi_grid(Prec prec);
 // queues (XCDs with an L2 of their own) of the XCD-cooperative two-step
 // kernel on this device; 0 = the kernel is not used here
 unsigned coop_queues();
-int launch_multi(Prec prec, bool inv, const MultiArgs& a, unsigned grid, hipStream_t st);
+// per_item: one work-group per item (grid = item count, no counter);
+// otherwise `grid` persistent work-groups claim items from a.counter, which
+// a one-lane kernel clears first
+int launch_multi(Prec prec, bool inv, const MultiArgs& a, unsigned grid, bool per_item, hipStream_t st);
 // schedule codes (Tune::rsplit) of the residue-split kernels instantiated
 // for complex transforms of length 2^logN whose LDS fits this device
 std::vector<int> rsplit_codes(Prec prec, int logN);

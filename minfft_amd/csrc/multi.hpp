@@ -11,7 +11,10 @@
 // (wavefront-local when a transform fits one wavefront).  Items of one
 // launch stream through every CU back to back, so the short groups pay one
 // launch and one ramp instead of one per length.  The counter is cleared by
-// a one-lane kernel queued in front of each launch.
+// a one-lane kernel queued in front of each launch.  The same kernel also
+// runs one item per work-group (no counter, grid = item count): faster when
+// the launch has the GPU to itself, while the persistent form leaves CUs to
+// other groups running beside it.
 #pragma once
 #include "fft_core.hpp"
 
@@ -70,12 +73,39 @@ __device__ __forceinline__ void multi_item(const MultiArgs& A, const MultiGroup&
     }
 }
 
+// item `item` of the launch: find its group, run it
+template <typename T, bool INV>
+__device__ __forceinline__ void multi_dispatch(const MultiArgs& A, unsigned item, c2<T>* lds) {
+    int gi = 0;
+    while (gi + 1 < A.ngroups && item >= A.g[gi + 1].item0) ++gi;
+    const MultiGroup& G = A.g[gi];
+    const unsigned li = item - G.item0;
+    switch (G.logL) {
+        case 4: multi_item<T, 4, INV>(A, G, li, lds); break;
+        case 5: multi_item<T, 5, INV>(A, G, li, lds); break;
+        case 6: multi_item<T, 6, INV>(A, G, li, lds); break;
+        case 7: multi_item<T, 7, INV>(A, G, li, lds); break;
+        case 8: multi_item<T, 8, INV>(A, G, li, lds); break;
+        case 9: multi_item<T, 9, INV>(A, G, li, lds); break;
+        case 10: multi_item<T, 10, INV>(A, G, li, lds); break;
+        case 11: multi_item<T, 11, INV>(A, G, li, lds); break;
+        default: multi_item<T, 12, INV>(A, G, li, lds); break;
+    }
+}
+
+// counter != nullptr: persistent, items claimed from the counter;
+// counter == nullptr: one item per work-group (item = work-group index,
+// the grid is the item count; no counter, no reset launch)
 template <typename T, bool INV>
 __global__ void __launch_bounds__(kMultiThreads, sizeof(T) == 8 ? 2 : 4) multi_kernel(MultiArgs A) {
     using C2 = c2<T>;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     C2* lds = reinterpret_cast<C2*>(smem_raw);
     __shared__ unsigned s_item[2];
+    if (A.counter == nullptr) {
+        if (blockIdx.x < A.total) multi_dispatch<T, INV>(A, blockIdx.x, lds);
+        return;
+    }
     for (int it = 0;; ++it) {
         // item broadcast through LDS; alternating cells, so that the write of
         // the next item never races with the read of this one
@@ -83,21 +113,7 @@ __global__ void __launch_bounds__(kMultiThreads, sizeof(T) == 8 ? 2 : 4) multi_k
         __syncthreads();  // also: every lane is done with the previous item's slots
         const unsigned item = s_item[it & 1];
         if (item >= A.total) break;  // uniform over the work-group
-        int gi = 0;
-        while (gi + 1 < A.ngroups && item >= A.g[gi + 1].item0) ++gi;
-        const MultiGroup& G = A.g[gi];
-        const unsigned li = item - G.item0;
-        switch (G.logL) {
-            case 4: multi_item<T, 4, INV>(A, G, li, lds); break;
-            case 5: multi_item<T, 5, INV>(A, G, li, lds); break;
-            case 6: multi_item<T, 6, INV>(A, G, li, lds); break;
-            case 7: multi_item<T, 7, INV>(A, G, li, lds); break;
-            case 8: multi_item<T, 8, INV>(A, G, li, lds); break;
-            case 9: multi_item<T, 9, INV>(A, G, li, lds); break;
-            case 10: multi_item<T, 10, INV>(A, G, li, lds); break;
-            case 11: multi_item<T, 11, INV>(A, G, li, lds); break;
-            default: multi_item<T, 12, INV>(A, G, li, lds); break;
-        }
+        multi_dispatch<T, INV>(A, item, lds);
     }
 }
 

@@ -256,11 +256,11 @@ def test_persistent_short_lengths(oracle64, prec, monkeypatch):
     din = torch.from_numpy(hin).cuda()
     mode = mf.PREC_FP64 if prec == 64 else mf.PREC_FP32
     outs = {}
-    for flag in ("1", "0"):
+    for flag in ("1", "2", "0"):
         monkeypatch.setenv("MINFFT_MULTI", flag)
         b = mf.Batch([(k, s, o, o) for (k, s), o in zip(entries, offs)], mode)
         desc = b.describe()
-        assert ("persistent" in desc) == (flag == "1"), desc
+        assert ("persistent" in desc) == (flag != "0"), desc
         dout = torch.zeros(pos + cs, dtype=torch.uint8, device="cuda")
         print(f"[case] persistent short lengths MINFFT_MULTI={flag} prec={prec}", flush=True)
         for _ in range(2):  # capture, then replay
@@ -272,7 +272,7 @@ def test_persistent_short_lengths(oracle64, prec, monkeypatch):
         kind, shape = entries[i]
         ref = oracle64.run(kind, xs[i].astype(np.complex128), shape)
         bar = tol64(shape) if prec == 64 else TOL32
-        for f in ("1", "0"):
+        for f in ("1", "2", "0"):
             y = outs[f][offs[i]:offs[i] + xs[i].nbytes].view(cdt)
             assert rel_l2(y, ref) <= bar, (f, i, kind, shape)
 
