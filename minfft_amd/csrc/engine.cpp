@@ -477,7 +477,14 @@ static int launch_tile(Prec prec, int kind, int logL, const void* tw, const void
     A.sc = sc;
     const unsigned long long tiles = (ntr + C - 1) / C;
     A.ntiles = (unsigned)tiles;
-    A.xcd = (tn.xcd && wants_cf && tiles >= 2ull * D.nxcd) ? D.nxcd : 1;
+    // XCD-grouped column tiles share their rows' lines in one L2; for
+    // strides of 1 MiB and more (the outer axis of a 512^3 array: 2 MiB)
+    // the grouping measured slower -- 512^3 split-FP16 outer pass 537 ->
+    // 516 us without it, the FP32 tile 475 -> 473 us, while the middle axis
+    // (4 KiB) keeps it (FP32 370 against 391 us without)
+    const long long far = (long long)(1 << 20) / (long long)cs;
+    const bool far_stride = (!in_row && A.in_e >= far) || (!out_row && A.out_e >= far);
+    A.xcd = (tn.xcd && wants_cf && !far_stride && tiles >= 2ull * D.nxcd) ? D.nxcd : 1;
     // wavefront-local core exchanges need a row-mapped core; staged column
     // I/O keeps its work-group barriers (tile.hpp)
     A.wave = (tn.wave && !A.cfast) ? 1 : 0;
