@@ -185,6 +185,42 @@ def config_copy(a, torch, mf):
             "bytes_read": [512 << 20, 256 << 20, 128 << 20], "bytes_written": [512 << 20, 256 << 20, 128 << 20]}
 
 
+def config_groups(a, torch, mf):
+    """Each length group of the default workload alone (its own plan, tuned
+    as in the batch, settled), then the whole batch: where the step's time
+    goes.  The short lengths (2^6..2^12) form one unit, as in the batch."""
+    sizes = workload()
+    offs = np.concatenate([[0], np.cumsum(sizes)[:-1]])
+    total = int(sizes.sum())
+    g = torch.Generator(device="cuda")
+    g.manual_seed(SEED)
+    xin = torch.view_as_complex((torch.rand(total, 2, generator=g, device="cuda") * 2 - 1).contiguous())
+    xout = torch.empty_like(xin)
+    st = torch.cuda.Stream()
+    sh = st.cuda_stream
+    groups = [[65536], [32768], [16384], [8192], [64, 128, 256, 512, 1024, 2048, 4096]]
+    rows = []
+    for grp in groups:
+        idx = np.nonzero(np.isin(sizes, grp))[0]
+        plan = mf.Batch([("dft", (int(sizes[i]),), int(offs[i]) * 8, int(offs[i]) * 8) for i in idx], mf.PREC_FP32)
+        ms = _time_enqueue(torch, lambda: plan.run(xin.data_ptr(), xout.data_ptr(), sh), a.steps, a.warmup, st)
+        nb = float(sizes[idx].sum()) * 16
+        rows.append({"lengths": grp if len(grp) > 1 else grp[0], "entries": int(len(idx)), "ms": round(ms, 5),
+                     "alg_bytes": nb, "frac": round(nb / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                     "schedule": [ln.split(": ", 1)[1] for ln in plan.describe().strip().split("\n")][0]})
+        log(f"[bench] group {rows[-1]['lengths']} x{len(idx)}: {ms * 1e3:.1f} us, frac {rows[-1]['frac']}")
+        plan.close()
+    plan = mf.Batch([("dft", (int(n),), int(o) * 8, int(o) * 8) for n, o in zip(sizes, offs)], mf.PREC_FP32)
+    ms = _time_enqueue(torch, lambda: plan.run(xin.data_ptr(), xout.data_ptr(), sh), a.steps, a.warmup, st)
+    plan.close()
+    alg = float(total) * 16
+    flops = float(np.sum(5.0 * sizes * np.log2(sizes)))
+    return _line(METRIC, round(flops / (ms * 1e-3) / 1e9, 3), "GFLOP/s", ms, a.steps, a.warmup, "f32",
+                 "default batch (8192 x 1D C2C FP32) and each of its length groups alone",
+                 {"groups": rows, "sum_of_groups_ms": round(sum(r["ms"] for r in rows), 5),
+                  "step_frac": round(alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)})
+
+
 def check_sample(xin, xout, sizes, offs, nsample=48):
     """rel-L2 and max-abs error vs the FP64 oracle on a spread of entries."""
     from oracle.oracle import Oracle
@@ -706,7 +742,7 @@ def main():
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison leg")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a 512 MiB device copy (calibrates the HBM byte counters)")
-    ap.add_argument("--config", default="c3", choices=["c1", "c2", "c3", "c4", "c5", "rlong", "copy"],
+    ap.add_argument("--config", default="c3", choices=["c1", "c2", "c3", "c4", "c5", "rlong", "copy", "groups"],
                     help="c3 (default) is the metric's workload; the others print their own line")
     ap.add_argument("--settle-ms", type=float, default=SETTLE_MS,
                     help="untimed replays of each timed workload before its warm-up steps, in ms (0 = none)")
@@ -722,7 +758,7 @@ def main():
 
     if a.config != "c3":
         fn = {"c1": config_c1, "c2": config_c2, "c4": config_c4, "c5": config_c5, "rlong": config_rlong,
-              "copy": config_copy}[a.config]
+              "copy": config_copy, "groups": config_groups}[a.config]
         print(json.dumps(fn(a, torch, mf)), flush=True)
         return
 
