@@ -247,30 +247,72 @@ def check_sample(xin, xout, sizes, offs, nsample=48):
             "rel_l2_pooled": math.sqrt(num / den), "max_abs": worst_abs}
 
 
-def cpu_baseline(sizes, budget_s=12.0):
-    """Reference minfft (oracle/_ref, FP32, -O3) on one host core, over the
-    workload's entries in order until the time budget is spent."""
+def _cpu_harness(ref):
+    """oracle/libcpubatch.so (a C timing loop over the reference's
+    functions on POSIX threads) bound to `ref`'s FP32 dft, or None if it is
+    not built."""
+    import ctypes
+    p = os.path.join(ROOT, "oracle", "libcpubatch.so")
+    if not os.path.exists(p):
+        return None
+    h = ctypes.CDLL(p)
+    f = h.cpu_batch_seconds
+    f.restype = ctypes.c_double
+    f.argtypes = [ctypes.POINTER(ctypes.c_int), ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                  ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+    addr = lambda name: ctypes.cast(ref.fn(name), ctypes.c_void_p).value  # noqa: E731
+
+    def run(sizes, threads, reps):
+        arr = (ctypes.c_int * len(sizes))(*[int(n) for n in sizes])
+        return f(arr, len(sizes), threads, reps, 8, addr("mkaux_dft"), addr("dft"), addr("free_aux"))
+    return run
+
+
+def _ref32():
     from oracle.oracle import Reference, ref_available
     flags = open("/proc/cpuinfo").read()
     fast = ref_available(32, fast=True) and " avx2 " in flags and " fma " in flags
     if not ref_available(32, fast=fast):
+        return None, fast
+    return Reference(32, fast=fast), fast
+
+
+def cpu_baseline(sizes, budget_s=12.0):
+    """Reference minfft (oracle/_ref, FP32, -O3) on one host core over the
+    workload's entries: the C harness (oracle/cpu_batch.c) runs the whole
+    workload on one thread, repeated to fill a few seconds; without it,
+    entries in order through ctypes until the time budget is spent."""
+    ref, fast = _ref32()
+    if ref is None:
         return None
-    ref = Reference(32, fast=fast)
+    opt = "-O3 -march=x86-64-v3" if fast else "-O2"
+    harness = _cpu_harness(ref)
+    if harness is not None:
+        flops = float(np.sum(5.0 * sizes * np.log2(sizes)))
+        t = harness(sizes, 1, 1)  # one pass (~0.5 s) sizes the timed run
+        if t > 0:
+            reps = max(1, int(math.ceil(3.0 / t)))
+            t = harness(sizes, 1, reps)
+        if t > 0:
+            return {"value": reps * flops / t / 1e9, "unit": "GFLOP/s", "cores": 1, "kind": "reference",
+                    "sample": f"the whole workload ({len(sizes)} entries) {reps} times on one thread, reference "
+                              f"minfft FP32 {opt} called from a C loop (oracle/cpu_batch.c), one aux per size"}
     rng = np.random.default_rng(SEED + 1)
     auxes = {}
     flops = 0.0
     spent = 0.0
     done = 0
     pts = 0
+    f = ref.fn("dft")
     for n in sizes:
         n = int(n)
         if n not in auxes:
             auxes[n] = ref.mkaux("dft", (n,))
         x = (rng.uniform(-1, 1, n) + 1j * rng.uniform(-1, 1, n)).astype(np.complex64)
         y = np.empty_like(x)
-        f = ref.fn("dft")
+        xp, yp, ap = x.ctypes.data, y.ctypes.data, auxes[n]
         t0 = time.perf_counter()
-        f(x.ctypes.data, y.ctypes.data, auxes[n])
+        f(xp, yp, ap)
         spent += time.perf_counter() - t0
         flops += 5.0 * n * math.log2(n)
         done += 1
@@ -281,26 +323,34 @@ def cpu_baseline(sizes, budget_s=12.0):
         ref.free(a)
     return {"value": flops / spent / 1e9, "unit": "GFLOP/s", "cores": 1, "kind": "reference",
             "sample": f"first {done} of {len(sizes)} entries ({pts} points), reference minfft "
-                      f"FP32 {'-O3 -march=x86-64-v3' if fast else '-O2'}, one transform at a time"}
+                      f"FP32 {opt}, one transform at a time through ctypes"}
 
 
 def cpu_baseline_all_cores(sizes, single, budget_s=4.0, max_threads=16):
     """Reference minfft (oracle/_ref, FP32) on the host's cores: one thread
     per core (at most 16, the host share of one GPU), each with its own
     auxes (a reference aux owns scratch), thread k taking entries k, k+T, ...
-    of the workload, repeated to fill about budget_s of wall time; ctypes
-    releases the GIL inside the calls."""
-    from concurrent.futures import ThreadPoolExecutor
-    from oracle.oracle import Reference, ref_available
+    of the workload, repeated to fill about budget_s of wall time -- in the
+    C harness (oracle/cpu_batch.c) when it is built, else in Python threads
+    (ctypes releases the GIL inside the calls)."""
     if not single:
         return None
-    fast = "-O3" in single["sample"]
-    if not ref_available(32, fast=fast):
+    ref, fast = _ref32()
+    if ref is None:
         return None
+    opt = "-O3 -march=x86-64-v3" if fast else "-O2"
     threads = max(1, min(max_threads, os.cpu_count() or 1))
     step_flops = float(sum(5.0 * int(n) * math.log2(int(n)) for n in sizes))
     reps = max(1, int(math.ceil(budget_s * threads * single["value"] * 1e9 / step_flops)))
-    ref = Reference(32, fast=fast)
+    harness = _cpu_harness(ref)
+    if harness is not None:
+        wall = harness(sizes, threads, reps)
+        if wall > 0:
+            return {"value": reps * step_flops / wall / 1e9, "unit": "GFLOP/s", "cores": threads,
+                    "kind": "reference",
+                    "sample": f"the whole workload {reps} times, entries split over {threads} threads, reference "
+                              f"minfft FP32 {opt} called from C (oracle/cpu_batch.c), one aux per size per thread"}
+    from concurrent.futures import ThreadPoolExecutor
     f = ref.fn("dft")
     nmax = int(max(sizes))
 
@@ -309,13 +359,14 @@ def cpu_baseline_all_cores(sizes, single, budget_s=4.0, max_threads=16):
         rng = np.random.default_rng(SEED + 2 + k)
         x = (rng.uniform(-1, 1, nmax) + 1j * rng.uniform(-1, 1, nmax)).astype(np.complex64)
         y = np.empty_like(x)
+        xp, yp = x.ctypes.data, y.ctypes.data
         mine = [int(n) for n in sizes[k::threads]]
         for n in set(mine):
             auxes[n] = ref.mkaux("dft", (n,))
         barrier.wait()
         for _ in range(reps):
             for n in mine:
-                f(x.ctypes.data, y.ctypes.data, auxes[n])
+                f(xp, yp, auxes[n])
         for a in auxes.values():
             ref.free(a)
 
@@ -330,7 +381,7 @@ def cpu_baseline_all_cores(sizes, single, budget_s=4.0, max_threads=16):
         wall = time.perf_counter() - t0
     return {"value": reps * step_flops / wall / 1e9, "unit": "GFLOP/s", "cores": threads, "kind": "reference",
             "sample": f"the whole workload {reps} times, entries split over {threads} threads, reference minfft "
-                      f"FP32 {'-O3 -march=x86-64-v3' if fast else '-O2'}, one aux per size per thread"}
+                      f"FP32 {opt} through ctypes in Python threads, one aux per size per thread"}
 
 
 # --------------------------------------------------------------------------

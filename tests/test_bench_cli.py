@@ -44,3 +44,20 @@ def test_default_workload():
     assert dict(zip(vals.tolist(), counts.tolist()))[65536] == 747
     # the same seed gives the same workload
     assert np.array_equal(sizes, bench.workload())
+
+
+def test_cpu_harness_times_the_reference():
+    """The C timing harness behind the CPU baselines runs the reference's
+    functions on threads and reports -1 on lengths it cannot run."""
+    import pytest
+    import bench
+    from oracle.oracle import Reference, ref_available
+
+    if not ref_available(32) or not os.path.exists(os.path.join(ROOT, "oracle", "libcpubatch.so")):
+        pytest.skip("oracle/_ref or oracle/libcpubatch.so not built")
+    run = bench._cpu_harness(Reference(32))
+    assert run([64, 1024, 256, 65536, 8], 1, 2) > 0
+    assert run([64, 1024, 256, 65536, 8] * 3, 4, 1) > 0
+    assert run([64, 1024], 8, 1) > 0  # more threads than entries
+    assert run([96], 1, 1) == -1.0  # not a power of two
+    assert run([64], 0, 1) == -1.0
