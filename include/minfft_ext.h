@@ -65,8 +65,11 @@ minfft_batch_plan *minfft_mkaux_batch(int count, const minfft_batch_entry *entri
    captures a HIP graph on the first run with a given (in, out, stream) and
    replays it later, its units on concurrent branches; a plan with one unit
    launches its kernels straight into the stream (MINFFT_GRAPH=1 / 0 forces
-   the graph / direct launches).  Asynchronous: returns 0 once enqueued, -1
-   on error. */
+   the graph / direct launches).  On a stream that the caller is capturing
+   into a graph of its own, the launches go into that capture (no graph of
+   the plan's own; run the plan once on the same buffers beforehand, so that
+   the capture makes no pointer queries or stream creations).  Asynchronous:
+   returns 0 once enqueued, -1 on error. */
 int minfft_batch_run(minfft_batch_plan *p, const void *in, void *out, void *stream);
 
 /* Same, but synchronous and accepting host or device pointers; in_bytes and

@@ -132,6 +132,13 @@ static int graph_mode() {
 static size_t item_in_bytes(int kind, Prec p) { return (kind_complex_in(kind) ? 2 : 1) * real_size(p); }
 static size_t item_out_bytes(int kind, Prec p) { return (kind_complex_out(kind) ? 2 : 1) * real_size(p); }
 
+// size limits of a batch: an entry as large as the plain makers accept, and
+// bounds on offsets and the points of all entries that no device buffer
+// reaches, so that the extent arithmetic below cannot overflow
+constexpr long long kMaxEntryPoints = 1LL << 40;
+constexpr long long kMaxOffset = 1LL << 50;
+constexpr long long kMaxBatchPoints = 1LL << 44;
+
 static int lg2(long long x) {
     int l = 0;
     while ((1LL << l) < x) ++l;
@@ -224,7 +231,6 @@ minfft_batch_plan* minfft_mkaux_batch(int count, const minfft_batch_entry* e, in
         set_error("unknown precision");
         return nullptr;
     }
-    if (!ensure_device()) return nullptr;
     minfft_batch_plan* p = new minfft_batch_plan();
     p->prec = prec == MINFFT_FP64 ? PREC_F64 : PREC_F32;  // split-FP16 keeps FP32 data
     p->prec_req = prec;
@@ -237,6 +243,7 @@ minfft_batch_plan* minfft_mkaux_batch(int count, const minfft_batch_entry* e, in
     std::map<std::tuple<int, int, int, int, int>, int> idx;
     int twlog = 1;
     size_t fs_max = 0;
+    long long total_points = 0;
     for (int i = 0; i < count; ++i) {
         const minfft_batch_entry& x = e[i];
         if (x.kind < 0 || x.kind >= K_COUNT) return fail("invalid entry kind");
@@ -246,11 +253,16 @@ minfft_batch_plan* minfft_mkaux_batch(int count, const minfft_batch_entry* e, in
         for (int k = 0; k < x.d; ++k) {
             if (x.n[k] <= 0 || (x.n[k] & (x.n[k] - 1))) return fail("entry lengths must be powers of two");
             n[k] = x.n[k];
+            if (P > kMaxEntryPoints / n[k]) return fail("entry too large");  // (as the plain makers)
             P *= n[k];
         }
         const size_t ib = item_in_bytes(x.kind, p->prec), ob = item_out_bytes(x.kind, p->prec);
         if (x.in_offset < 0 || x.out_offset < 0 || x.in_offset % (long long)ib || x.out_offset % (long long)ob)
             return fail("entry offsets must be non-negative multiples of the item size");
+        if (x.in_offset > kMaxOffset || x.out_offset > kMaxOffset)
+            return fail("entry offset beyond any device buffer");
+        total_points += P;
+        if (total_points > kMaxBatchPoints) return fail("batch too large");
         auto key = std::make_tuple(x.kind, x.d, n[0], n[1], n[2]);
         auto it = idx.find(key);
         int gi;
@@ -274,6 +286,7 @@ minfft_batch_plan* minfft_mkaux_batch(int count, const minfft_batch_entry* e, in
         p->in_extent = std::max(p->in_extent, (size_t)x.in_offset + (size_t)ii * ib);
         p->out_extent = std::max(p->out_extent, (size_t)x.out_offset + (size_t)oi * ob);
     }
+    if (!ensure_device()) return fail(nullptr);  // (entries are checked first, without a device)
     p->ws = Workspace();
     // blocking: ordered after the legacy default stream's earlier work (as the plain aux's stream)
     if (hipStreamCreateWithFlags(&p->stream, hipStreamDefault) != hipSuccess) return fail("hipStreamCreate failed");
@@ -290,6 +303,7 @@ minfft_batch_plan* minfft_mkaux_batch(int count, const minfft_batch_entry* e, in
         for (int a = 0; a < g.d; ++a) {
             int kind = g.kind;
             if (realfam && a < g.d - 1) kind = K_DFT;
+            if (const char* why = axis_length_error(p->prec, kind, lg2(g.n[a]))) return fail(why);
             twlog = std::max(twlog, needed_tw_log(kind, lg2(g.n[a])));
             const long long others = (realfam && a < g.d - 1) ? rows / g.n[a] * H : P / g.n[a];
             fs = std::max(fs, axis_scratch_bytes(p->prec, kind, lg2(g.n[a]), (unsigned long long)others * cnt,
@@ -520,6 +534,16 @@ int minfft_batch_run(minfft_batch_plan* p, const void* in, void* out, void* stre
     for (const BatchGroup& g : p->groups) units += g.in_multi ? 0 : 1;
     const int gm = graph_mode();
     if (gm == 0 || (gm < 0 && units <= 1)) return enqueue_batch(p, in, out, st, branch_count());
+    // a caller capturing `st` into a graph of its own gets the launches in
+    // that capture (the branches become branches of the caller's graph):
+    // a capture of our own cannot begin on a stream that is capturing
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(st, &cap) != hipSuccess) {
+        (void)hipGetLastError();
+        set_error("hipStreamIsCapturing failed");
+        return -1;
+    }
+    if (cap != hipStreamCaptureStatusNone) return enqueue_batch(p, in, out, st, branch_count());
     if (!(p->exec && p->g_in == in && p->g_out == out && p->g_st == st)) {
         if (p->exec) (void)hipGraphExecDestroy(p->exec);
         if (p->graph) (void)hipGraphDestroy(p->graph);

@@ -522,6 +522,16 @@ int needed_tw_log(int kind, int logN) {
     return std::max(logN + 3, 1);
 }
 
+const char* axis_length_error(Prec prec, int kind, int logN) {
+    if (!ensure_device()) return "no GPU device";
+    const int maxlog = device().maxlog[prec];
+    // the complex FFT the pass runs: the axis itself, or half of it for the
+    // real, DCT and DST kinds
+    const int lc = (kind == K_DFT || kind == K_INVDFT) ? logN : logN - 1;
+    if (lc > maxlog && lc - lc / 2 > maxlog) return "transform length exceeds the two-step limit";
+    return nullptr;
+}
+
 size_t axis_scratch_bytes(Prec prec, int kind, int logN, unsigned long long ntrans,
                           unsigned long long in_extent) {
     if (!ensure_device()) return 0;
@@ -1071,11 +1081,11 @@ Plan* make_plan(Family fam, Prec prec, int d, const int* Ns) {
             set_error("transform lengths must be positive powers of two");
             return nullptr;
         }
-        P *= N;
-        if (P > (1LL << 40)) {
+        if (P > (1LL << 40) / N) {  // (checked before the product, which could overflow)
             set_error("transform too large");
             return nullptr;
         }
+        P *= N;
     }
     if (!ensure_device()) return nullptr;
     Plan* p = new Plan();
@@ -1103,7 +1113,6 @@ Plan* make_plan(Family fam, Prec prec, int d, const int* Ns) {
     family_kinds(fam, kinds, nk);
     int twlog = 1;
     size_t fs = 0;
-    const int maxlog = device().maxlog[prec];
     const long long Nd = Ns[d - 1];
     const long long H = Nd / 2 + 1;
     const long long rows = P / Nd;
@@ -1111,19 +1120,12 @@ Plan* make_plan(Family fam, Prec prec, int d, const int* Ns) {
         const int logN = ilog2i(Ns[a]);
         int kind = kinds[0];
         if (fam == FAM_REALDFT && a < d - 1) kind = K_DFT;
+        if (const char* why = axis_length_error(prec, kind, logN)) return fail(why);
         twlog = std::max(twlog, needed_tw_log(kind, logN));
         const long long others = (fam == FAM_REALDFT && a < d - 1) ? rows / Ns[a] * H : P / Ns[a];
         const long long extent = (fam == FAM_REALDFT && a < d - 1) ? rows * H : P;
         fs = std::max(fs, axis_scratch_bytes(prec, kind, logN, (unsigned long long)others,
                                              (unsigned long long)extent));
-        if (kind == K_DFT && logN > maxlog) {
-            const int l2 = logN - logN / 2;
-            if (l2 > maxlog) return fail("transform length exceeds the two-step limit");
-        }
-        if (kind != K_DFT && logN >= 1 && logN - 1 > maxlog) {
-            const int lL = logN - 1, l2 = lL - lL / 2;
-            if (l2 > maxlog) return fail("transform length exceeds the two-step limit");
-        }
     }
     (void)cs;
     if (alloc_twiddles(p->ws, prec, twlog, p->stream)) return fail(nullptr);

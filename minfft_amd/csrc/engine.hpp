@@ -188,6 +188,9 @@ int execute(Plan& p, int kind, const void* in, void* out);
 int alloc_twiddles(Workspace& ws, Prec prec, int log_m, hipStream_t st);
 void free_workspace(Workspace& ws);
 int needed_tw_log(int kind, int logN);
+// nullptr if a `kind` pass along an axis of 2^logN points fits the two-step
+// decomposition at this precision, else the reason
+const char* axis_length_error(Prec prec, int kind, int logN);
 // one 1D pass of `kind` (length N = 2^logN along the axis) over a batch
 int axis_pass(const Workspace& ws, Prec prec, int kind, int logN,
               const void* in, long long in_e, void* out, long long out_e,

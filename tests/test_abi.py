@@ -118,3 +118,25 @@ def test_makers_fail_loudly_without_a_device():
         assert not lib.mkaux_raw("minfft_mkaux_t2t3_2d", 8, 8)
     with pytest.raises(mf.MinfftError, match="device"):
         mf.Batch([("dft", (8,), 0, 0)], mf.PREC_FP32)
+
+
+def test_size_checks_come_before_any_device_call():
+    """Sizes that no device can hold are refused by the makers themselves,
+    before any HIP call, so the checks hold with no GPU: a product of axis
+    lengths beyond 2^40 points (including one that would wrap a 64-bit
+    product: 2^20 x 2^20 x 2^30), batch entries the same, entry offsets
+    beyond any buffer, and batches whose entries hold more than 2^44 points
+    in all."""
+    for prec in (64, 32):
+        lib = mf.Minfft(prec)
+        assert not lib.mkaux_raw("minfft_mkaux_dft_3d", 1 << 20, 1 << 20, 1 << 30)
+        assert "too large" in mf.last_error(), mf.last_error()
+        assert not lib.mkaux_raw("minfft_mkaux_realdft_3d", 1 << 14, 1 << 14, 1 << 14)
+        assert "too large" in mf.last_error(), mf.last_error()
+    for dims in [(1 << 20, 1 << 20, 1 << 30), (1 << 14, 1 << 14, 1 << 14)]:
+        with pytest.raises(mf.MinfftError, match="entry too large"):
+            mf.Batch([("dft", (8,), 0, 0), ("dft", dims, 0, 0)], mf.PREC_FP32)
+    with pytest.raises(mf.MinfftError, match="offset beyond"):
+        mf.Batch([("dft", (8,), 8 << 50, 0)], mf.PREC_FP32)
+    with pytest.raises(mf.MinfftError, match="batch too large"):
+        mf.Batch([("dft", (1 << 20, 1 << 20), 0, 0)] * 17, mf.PREC_FP64)

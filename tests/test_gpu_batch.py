@@ -319,6 +319,76 @@ def test_batch_allocation_failure_returns_null_without_leak():
     assert free1 >= free0 - (64 << 20), (free0, free1)
 
 
+@pytest.mark.parametrize("prec,logn", [(mf.PREC_FP64, 27), (mf.PREC_FP32, 29)])
+def test_batch_refuses_lengths_beyond_the_two_step_limit(prec, logn, monkeypatch):
+    """Batch entries longer than the plain makers accept (1D FP64 2^26, FP32
+    2^28; real kinds twice that) are refused when the plan is made, with the
+    plain makers' reason, before anything is allocated or tuned."""
+    _warm_device()
+    for kind, n in [("dft", 1 << logn), ("invdft", 1 << logn), ("realdft", 1 << (logn + 1)),
+                    ("dct2", 1 << (logn + 1))]:
+        print(f"[case] batch refuses {kind} 2^{n.bit_length() - 1} prec={prec}", flush=True)
+        with pytest.raises(mf.MinfftError, match="two-step limit"):
+            mf.Batch([("dft", (64,), 0, 0), (kind, (n,), 0, 0)], prec)
+    # the longest accepted complex length still makes a plan (untuned: no
+    # data is touched; the real kinds' longest plans would need twiddle
+    # tables of 8N entries, ~200 GB for FP32, so they are not made here)
+    monkeypatch.setenv("MINFFT_AUTOTUNE", "0")
+    b = mf.Batch([("dft", (1 << (logn - 1),), 0, 0)], prec)
+    b.close()
+
+
+def test_batch_inside_a_callers_graph_capture(oracle64):
+    """A plan with several units of work run on a stream that the caller is
+    capturing (torch.cuda.graph) puts its launches, branches included, into
+    the caller's graph; every replay, with the plan's own eager runs in
+    between, equals the eager result bit for bit, and sampled entries match
+    the oracle."""
+    torch = _torch()
+    spec = [("dft", 65536)] * 3 + [("dft", 16384)] * 4 + [("dft", 1024)] * 20 + [("dft", 64)] * 30 + \
+        [("invdft", 256)] * 10
+    entries, off = [], 0
+    for kind, n in spec:
+        entries.append((kind, (n,), off, off))
+        off += n * 8
+    npts = off // 8
+    g0 = torch.Generator(device="cuda")
+    g0.manual_seed(11)
+    x = torch.complex(torch.rand(npts, generator=g0, device="cuda") * 2 - 1,
+                      torch.rand(npts, generator=g0, device="cuda") * 2 - 1)
+    b = mf.Batch(entries, mf.PREC_FP32)
+    desc = b.describe()
+    own = [ln for ln in desc.splitlines() if "persistent" not in ln]
+    assert "persistent" in desc and len(own) == 2, desc  # four units: a graph of its own
+    s = torch.cuda.Stream()
+    want = torch.zeros_like(x)
+    b.run(x.data_ptr(), want.data_ptr(), s.cuda_stream)
+    y = torch.zeros_like(x)
+    b.run(x.data_ptr(), y.data_ptr(), s.cuda_stream)  # the pointers checked, the branch streams made
+    torch.cuda.synchronize()
+    print("[case] batch captured into a caller's graph", flush=True)
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph, stream=s):
+        b.run(x.data_ptr(), y.data_ptr(), s.cuda_stream)
+    torch.cuda.synchronize()
+    for r in range(4):
+        y.zero_()
+        torch.cuda.synchronize()
+        graph.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(y, want), r
+        z = torch.zeros_like(x)
+        b.run(x.data_ptr(), z.data_ptr(), None)  # the plan's own graph, eagerly
+        torch.cuda.synchronize()
+        assert torch.equal(z, want), r
+    b.close()
+    xh, yh = x.cpu().numpy(), want.cpu().numpy()
+    for i in (0, 3, 7, 27, 57, len(spec) - 1):
+        kind, (n,), o, _ = entries[i]
+        ref = oracle64.run(kind, xh[o // 8:o // 8 + n].astype(np.complex128), (n,))
+        assert rel_l2(yh[o // 8:o // 8 + n], ref) <= TOL32, (i, kind, n)
+
+
 @pytest.mark.parametrize("single_unit", [False, True], ids=["several-units", "one-unit"])
 def test_graph_replay_and_direct_launch_agree(oracle64, monkeypatch, single_unit):
     """The same plan run as a replayed graph (MINFFT_GRAPH=1), as direct
