@@ -1518,8 +1518,22 @@ int execute(Plan& p, int kind, const void* in, void* out) {
         set_error("aux data does not match the transform kind");
         return -1;
     }
-    int cur = -1;
-    if (hipGetDevice(&cur) != hipSuccess || cur != p.dev) (void)hipSetDevice(p.dev);
+    // the aux's device for this call; the caller's current device is
+    // restored on return
+    struct DeviceScope {
+        int prev = -1;
+        explicit DeviceScope(int dev) {
+            if (hipGetDevice(&prev) != hipSuccess) {
+                (void)hipGetLastError();
+                prev = -1;
+            }
+            if (prev != dev) (void)hipSetDevice(dev);
+        }
+        ~DeviceScope() {
+            int now = -1;
+            if (prev >= 0 && hipGetDevice(&now) == hipSuccess && now != prev) (void)hipSetDevice(prev);
+        }
+    } scope(p.dev);
     const size_t ib = in_bytes(p, kind), ob = out_bytes(p, kind);
     const bool in_host = !is_device_pointer(in), out_host = !is_device_pointer(out);
     if (!in_host && !out_host) return run_and_wait(p, kind, in, out);
