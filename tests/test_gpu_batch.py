@@ -389,6 +389,44 @@ def test_batch_inside_a_callers_graph_capture(oracle64):
         assert rel_l2(yh[o // 8:o // 8 + n], ref) <= TOL32, (i, kind, n)
 
 
+@pytest.mark.parametrize("prec", [32, 64])
+def test_uneven_batch_in_place_equals_out_of_place(oracle64, prec):
+    """An uneven 1D C2C batch like the benchmark's (2^6..2^16, forward and
+    inverse, every kernel of the batch: the short-length launch, tiles,
+    radix-2-split and fused two-step kernels) run in place gives the same
+    bits as out of place, and sampled entries match the oracle."""
+    torch = _torch()
+    rng = np.random.default_rng(77)
+    lens = [1 << int(6 + 11 * rng.random()) for _ in range(420)] + [1 << 16] * 270
+    kinds = ["dft" if rng.random() < 0.7 else "invdft" for _ in lens]
+    entries, off = [], 0
+    for k, n in zip(kinds, lens):
+        entries.append((k, (n,), off, off))
+        off += n * (16 if prec == 64 else 8)
+    npts = sum(lens)
+    rdt = torch.float64 if prec == 64 else torch.float32
+    g = torch.Generator(device="cuda")
+    g.manual_seed(prec)
+    x = torch.complex(torch.rand(npts, generator=g, device="cuda", dtype=rdt) * 2 - 1,
+                      torch.rand(npts, generator=g, device="cuda", dtype=rdt) * 2 - 1)
+    b = mf.Batch(entries, mf.PREC_FP64 if prec == 64 else mf.PREC_FP32)
+    print(f"[case] in-place uneven batch prec={prec}: {len(entries)} entries\n{b.describe()}", flush=True)
+    y = torch.zeros_like(x)
+    b.run(x.data_ptr(), y.data_ptr(), None)
+    z = x.clone()
+    b.run(z.data_ptr(), z.data_ptr(), None)
+    torch.cuda.synchronize()
+    b.close()
+    assert torch.equal(z, y)
+    isz = 16 if prec == 64 else 8
+    xh, yh = x.cpu().numpy(), y.cpu().numpy()
+    for i in (0, 1, 2, 100, 419, 420, len(entries) - 1):
+        kind, (n,), o, _ = entries[i]
+        ref = oracle64.run(kind, xh[o // isz:o // isz + n].astype(np.complex128), (n,))
+        e = rel_l2(yh[o // isz:o // isz + n], ref)
+        assert e <= (tol64((n,)) if prec == 64 else TOL32), (i, kind, n, e)
+
+
 @pytest.mark.parametrize("single_unit", [False, True], ids=["several-units", "one-unit"])
 def test_graph_replay_and_direct_launch_agree(oracle64, monkeypatch, single_unit):
     """The same plan run as a replayed graph (MINFFT_GRAPH=1), as direct
