@@ -42,7 +42,7 @@ def test_oracle_matches_reference_golden_vectors(prec):
 
 @pytest.mark.skipif(not ref_available(64), reason="oracle/_ref not built")
 @pytest.mark.parametrize("prec", [64, 32])
-@pytest.mark.parametrize("shape", [(2048,), (64, 32), (4, 8, 16)])
+@pytest.mark.parametrize("shape", [(2048,), (64, 32), (4, 8, 16), (2, 4, 8, 2), (2, 1, 4, 2, 8)])
 def test_oracle_matches_compiled_reference(prec, shape):
     o, r = Oracle(prec), Reference(prec)
     for i, kind in enumerate(KINDS):
@@ -50,7 +50,8 @@ def test_oracle_matches_compiled_reference(prec, shape):
         assert rel_l2(o.run(kind, x, shape), r.run(kind, x, shape)) <= (1e-15 if prec == 64 else 1e-7)
 
 
-SHAPES = [(1,), (2,), (8,), (16,), (512,), (4096,), (8, 16), (32, 1), (4, 2, 8)]
+SHAPES = [(1,), (2,), (8,), (16,), (512,), (4096,), (8, 16), (32, 1), (4, 2, 8), (2, 4, 8, 2), (2, 2, 1, 4, 4),
+          (1, 2, 1, 4, 2, 2)]
 
 
 @pytest.mark.parametrize("shape", SHAPES)
