@@ -81,7 +81,8 @@ void minfft_free_batch(minfft_batch_plan *p);
 /* Replace group `group`'s schedule (groups are numbered as listed by
    minfft_batch_describe) by the process-wide one with the "key=value,..."
    changes in `spec` (keys tpw, cfb, cfcore, wave, xcd, e32, e4, split,
-   split2, fused, wpc, falt, rsplit, coop, coopla, mall, rlong, mstages;
+   split2, fused, wpc, falt, rsplit, coop, coopla, mall, rlong, mstages,
+   csplit;
    NULL or "" = the process-wide schedule).  The next run re-captures the
    graph.  A group that runs in the batch's persistent launch of short 1D
    complex groups (described as "persistent") keeps the schedule unused.

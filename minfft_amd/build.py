@@ -46,7 +46,7 @@ CLANGXX = os.environ.get("MINFFT_CLANGXX", "/opt/rocm/llvm/bin/clang++")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 
 HEADERS = ["fft_core.hpp", "kinds.hpp", "tile.hpp", "ktable.hpp", "engine.hpp", "big.hpp", "bigtable.hpp", "mfma.hpp",
-           "autotune.hpp", "split2.hpp", "rsplit.hpp", "multi.hpp", "coop.hpp", "rlong.hpp"]
+           "autotune.hpp", "split2.hpp", "rsplit.hpp", "multi.hpp", "coop.hpp", "rlong.hpp", "csplit.hpp"]
 
 
 def _newer(target, deps):
@@ -65,6 +65,7 @@ def _units():
             u.append((o, "inst.hip", [f"-DINST_F64={f64}", f"-DINST_KIND={kind}"], "engine"))
     u.append((os.path.join(OBJ, "twiddle.o"), "twiddle.hip", [], "engine"))
     u.append((os.path.join(OBJ, "big_inst.o"), "big_inst.hip", [], "engine"))
+    u.append((os.path.join(OBJ, "csplit_inst.o"), "csplit_inst.hip", [], "engine"))
     for part in range(3):
         u.append((os.path.join(OBJ, f"rsplit_inst{part}.o"), "rsplit_inst.hip", [f"-DRSPLIT_PART={part}"], "engine"))
     u.append((os.path.join(OBJ, "engine.o"), "engine.cpp", ["-x", "hip"], "engine"))

@@ -59,7 +59,7 @@ std::string key_of(Prec prec, int kind, int d, const int* n, unsigned count) {
 
 // Cache file: a header line naming the device, its CU count and the
 // library build, then one line per group:
-//   key tpw cfb cfcore wave xcd e32 e4 split split2 fused wpc falt rsplit coop coopla mall us_default us_best
+//   key tpw cfb cfcore wave xcd e32 e4 split split2 fused wpc falt rsplit coop coopla mall csplit us_default us_best
 // A file whose header names another device or build is ignored (and
 // replaced on the next save).  Saves rewrite the whole file into a
 // temporary one and rename it into place, so a reader never sees a partial
@@ -80,7 +80,7 @@ std::string cache_header() {
     std::string build;
     for (const char* c = b; *c; ++c)
         if (*c != ' ') build += *c;
-    std::snprintf(b, sizeof(b), "#minfft-tune v4 arch=%s cus=%d build=%s", arch.c_str(), ncu, build.c_str());
+    std::snprintf(b, sizeof(b), "#minfft-tune v5 arch=%s cus=%d build=%s", arch.c_str(), ncu, build.c_str());
     return b;
 }
 
@@ -99,7 +99,7 @@ void load_cache() {
         r.sched = base_tune();
         Tune& t = r.sched;
         if (ss >> k >> t.tpw >> t.cfb >> t.cfcore >> t.wave >> t.xcd >> t.e32 >> t.e4 >> t.split >> t.split2 >>
-            t.fused >> t.fused_wpc >> t.fused_alt >> t.rsplit >> t.coop >> t.coop_la >> t.mall_mb >>
+            t.fused >> t.fused_wpc >> t.fused_alt >> t.rsplit >> t.coop >> t.coop_la >> t.mall_mb >> t.csplit >>
             r.us_default >> r.us_best) {
             r.tuned = true;
             r.cached = true;
@@ -123,7 +123,7 @@ void save_cache() {
             out << kv.first << ' ' << t.tpw << ' ' << t.cfb << ' ' << t.cfcore << ' ' << t.wave << ' ' << t.xcd << ' '
                 << t.e32 << ' ' << t.e4 << ' ' << t.split << ' ' << t.split2 << ' ' << t.fused << ' ' << t.fused_wpc
                 << ' ' << t.fused_alt << ' ' << t.rsplit << ' ' << t.coop << ' ' << t.coop_la << ' ' << t.mall_mb
-                << ' ' << r.us_default
+                << ' ' << t.csplit << ' ' << r.us_default
                 << ' ' << r.us_best << '\n';
         }
         if (!out.good()) {
@@ -211,6 +211,7 @@ TuneResult autotune_group(const Workspace& ws, const TuneRequest& r, hipStream_t
         knobs.push_back({&Tune::cfb, {128, 64, 256, 32}});
         knobs.push_back({&Tune::xcd, {1, 0}});
     }
+    if (multi && (r.kind == K_DFT || r.kind == K_INVDFT)) knobs.push_back({&Tune::csplit, {-1, 0, 1}});
     {
         // radix-32 tiles and radix-2-split halves (FP32): FFT lengths of
         // 2^9 .. 2^(maxlog+1) points (a real kind's FFT is half its length)
@@ -353,12 +354,12 @@ TuneResult autotune_group(const Workspace& ws, const TuneRequest& r, hipStream_t
 }
 
 std::string describe_tune(const Tune& t) {
-    char b[224];
+    char b[256];
     std::snprintf(b, sizeof(b),
                   "tpw=%d cfb=%d cfcore=%d wave=%d xcd=%d e32=%d split=%d split2=%d fused=%d wpc=%d falt=%d rsplit=%d "
-                  "coop=%d coopla=%d mall=%d",
+                  "coop=%d coopla=%d mall=%d csplit=%d",
                   t.tpw, t.cfb, t.cfcore, t.wave, t.xcd, t.e32, t.split, t.split2, t.fused, t.fused_wpc, t.fused_alt,
-                  t.rsplit, t.coop, t.coop_la, t.mall_mb);
+                  t.rsplit, t.coop, t.coop_la, t.mall_mb, t.csplit);
     return b;
 }
 

@@ -45,6 +45,10 @@ struct BigTable {
     const void* multi[2][2];  // [prec][inverse]
     size_t multi_lds[2];
     const void* multi_reset;
+    // radix-2-split column tiles (csplit.hpp), by log2 L
+    const void* csplit[2][17][2][2];  // [prec][log2 L][non-temporal][inverse]
+    size_t csplit_slot[2][17];        // LDS bytes per transform
+    int csplit_tpt[2][17];            // lanes per transform (512 lanes per work-group at most)
 };
 
 void register_big(BigTable& t);
@@ -56,5 +60,6 @@ inline void register_rsplit(BigTable& t) {
     register_rsplit_part1(t);
     register_rsplit_part2(t);
 }
+void register_csplit(BigTable& t);  // csplit_inst.hip
 
 }  // namespace mfx

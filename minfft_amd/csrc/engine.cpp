@@ -92,6 +92,7 @@ static const BigTable& bigtable() {
         std::memset(&g_big, 0, sizeof(g_big));
         register_big(g_big);
         register_rsplit(g_big);
+        register_csplit(g_big);
     });
     return g_big;
 }
@@ -183,7 +184,8 @@ bool is_device_pointer(const void* p) {
 // ------------------------------------------------------------ tile tuning
 // Process-wide schedule: defaults, then MINFFT_TUNE="key=value,..." (keys:
 // tpw, cfb, cfcore, wave, xcd, e32, e4, stage, chunk, split, split2, fused,
-// wpc, falt, rsplit, coop, coopla, mall -- the fields of Tune in engine.hpp).
+// wpc, falt, rsplit, coop, coopla, mall, rlong, mstages, csplit -- the fields
+// of Tune in engine.hpp).
 static Tune g_tune;
 static std::once_flag g_tune_once;
 static thread_local const Tune* t_tune = nullptr;
@@ -225,6 +227,7 @@ bool parse_tune(const char* spec, Tune& t) {
         else if (k == "mall" && v >= 0) t.mall_mb = v;
         else if (k == "rlong") t.rlong = v;
         else if (k == "mstages" && (v == 1 || v == 2)) t.mstages = v;
+        else if (k == "csplit") t.csplit = v;
         else ok = false;
     }
     return ok;
@@ -428,8 +431,51 @@ static int launch_tile(Prec prec, int kind, int logL, const void* tw, const void
         A.lgB[3] = (A.B[3] & (A.B[3] - 1)) == 0 ? ilog2i((int)A.B[3]) : -1;
         A.ntrans = (unsigned)ntr;
     }
-    const bool wants_cf = A.cfast || A.in_mode == IO_STAGE_CF || A.out_mode == IO_STAGE_CF;
     const size_t cs = prec == PREC_F64 ? 16 : 8;
+    // radix-2-split column tile (csplit.hpp): both sides are columns of
+    // adjacent transforms with direct c-fast I/O; half of each transform in
+    // the LDS at a time, so a work-group takes twice the columns of the
+    // whole-transform tile (twice the row segment) in the same LDS.  By
+    // default for FP64 column passes and for FP32 ones with strides of
+    // 1 MiB and more; 512^3 per pass (kernel trace): FP32 outer axis (2 MiB)
+    // 473 -> 433 us, middle axis (4 KiB) 370 -> 390 us (so kept off there);
+    // FP64 outer 987 -> 847 us, middle 750 -> 723 us
+    {
+        const BigTable& bt = bigtable();
+        const int pi = prec == PREC_F64 ? 0 : 1, iv = kind == K_INVDFT;
+        const long long far_e = (long long)(1 << 20) / (long long)cs;
+        const bool cs_on = tn.csplit > 0 || (tn.csplit < 0 && (prec == PREC_F64 || in_e >= far_e || out_e >= far_e));
+        if (cs_on && cplx && !split16 && !e32 && !e4 && fs_logN == 0 && fn == kt.tile[kind][logL] &&
+            A.cfast && A.in_mode == IO_DIRECT && A.out_mode == IO_DIRECT && in_cf && out_cf && !in_row &&
+            !out_row && bt.csplit[pi][logL][0][iv]) {
+            const int tpt = bt.csplit_tpt[pi][logL];
+            const size_t slot = bt.csplit_slot[pi][logL];
+            int C = std::max(1, (int)(2 * (size_t)tn.cfb / cs));
+            while (C > 1 && C * tpt > 512) C >>= 1;  // the kernel's launch bound
+            if ((unsigned long long)C > ntr) C = (int)ntr;
+            while (C > 1 && (C / 2) * tpt >= 64 && (ntr + C - 1) / C < 2ull * (unsigned long long)D.ncu) C >>= 1;
+            while (C > 1 && (size_t)C * slot > (size_t)D.lds_max) C >>= 1;
+            while (C & (C - 1)) C &= C - 1;
+            const bool nt = ntr * (unsigned long long)L * cs >= kStreamBytes && (size_t)C * cs >= 128;
+            A.C = C;
+            A.lgC = ilog2i(C);
+            A.sc = slot_len(L / 2);
+            const unsigned long long tiles = (ntr + C - 1) / C;
+            A.ntiles = (unsigned)tiles;
+            const bool far_stride = in_e >= far_e || out_e >= far_e;
+            A.xcd = (tn.xcd && !far_stride && tiles >= 2ull * D.nxcd) ? D.nxcd : 1;
+            A.wave = 0;
+            if (t_signal && !t_signal->used && tiles == 1) {
+                A.hflag = t_signal->flag;
+                A.hval = t_signal->value;
+                t_signal->used = true;
+            }
+            void* args[] = {&A};
+            return launch_fn(bt.csplit[pi][logL][nt ? 1 : 0][iv], (unsigned)tiles, (unsigned)(C * tpt), args,
+                             (size_t)C * slot, st);
+        }
+    }
+    const bool wants_cf = A.cfast || A.in_mode == IO_STAGE_CF || A.out_mode == IO_STAGE_CF;
     const int sc = slot_len(L);
     const int maxT = (prec == PREC_F64 || e32 || split16) ? 512 : 1024;  // the kernels' launch bounds
     int C;

@@ -107,6 +107,10 @@ struct Tune {
     int rlong = 1;      // real/DCT/DST axes longer than a tile: 1 = two passes with the
                         // pre- and post-processing fused into the two-step FFT (rlong.hpp),
                         // 0 = pre-kernel, two-step FFT, post-kernel
+    int csplit = -1;    // complex column passes (both sides columns of adjacent transforms)
+                        // on the radix-2-split column tile (csplit.hpp) where instantiated:
+                        // 1 = on, 0 = off (whole-transform tile), -1 = FP64 passes and FP32
+                        // ones with strides of 1 MiB and more (the measured best)
 };
 struct Plan {
     unsigned magic = 0x6d667870u;

@@ -56,7 +56,7 @@ def check(kind, shape, prec, y, x, oracle64):
 SHAPES = [(1,), (2,), (4,), (8,), (16,), (32,), (64,), (128,), (256,), (512,), (1024,), (2048,),
           (4096,), (8192,), (16384,),
           (2, 2), (4, 8), (8, 1), (1, 8), (64, 32), (32, 64), (256, 256), (2, 4, 8), (8, 8, 8),
-          (16, 1, 32), (32, 16, 8),
+          (16, 1, 32), (32, 16, 8), (512, 32), (1024, 16), (512, 2, 8),
           # the generic makers take any number of dimensions (reference minfft.h:46-61)
           (2, 4, 8, 2), (8, 16, 4, 32), (2, 2, 1, 4, 4), (1, 2, 1, 4, 2, 2)]
 

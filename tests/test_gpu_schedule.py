@@ -20,7 +20,7 @@ SPECS = ["", "tpw=128", "tpw=512", "cfb=64", "cfb=256", "xcd=0", "wave=0",
          "fused=0,split=-1", "fused=0,split=1",
          "rsplit=11", "rsplit=12", "rsplit=21", "rsplit=22", "rsplit=31", "rsplit=32", "rsplit=41", "e4=0",
          "coop=0", "coop=1", "coop=1,coopla=1", "coop=1,coopla=4", "coop=1,coopla=16", "coop=1,coopla=64",
-         "mall=0", "mall=1", "mall=2", "mall=32"]
+         "mall=0", "mall=1", "mall=2", "mall=32", "csplit=0", "csplit=1", "csplit=1,cfb=64", "csplit=1,xcd=0"]
 
 
 class Packed:
@@ -85,6 +85,11 @@ def _cases(prec):
                 ("dst2", (1024,)), ("dct3", (2048,)), ("dst4", (1024, 4))] * 2
     entries += [("dft", (split_n // 2,))] * 6 + [("invdft", (split_n // 4,))] * 6
     entries += [("realdft", (1 << 17,))] * 3 + [("realdft", (2 * split_n,))] * 2
+    # complex column passes of 512 and 1024 points (the radix-2-split column
+    # tile; 3 x 16 columns leave a partial last tile of 32), and along axis 0
+    # of a 3D entry
+    entries += [("dft", (512, 64))] * 3 + [("invdft", (512, 16))] * 3 + [("dft", (1024, 32))] * 2
+    entries += [("invdft", (512, 8, 16))] * 2 + [("realdft", (512, 4, 32))] * 2
     # one-entry multi-dimensional groups (run as single transforms: slabs of axis 0)
     entries += [("dft", (64, 32, 16)), ("dct2", (128, 64)), ("invdft", (16, 2048)), ("dst4", (4, 8, 32))]
     xs = [make_input(k, s, prec, int(rng.integers(1 << 30))) for k, s in entries]

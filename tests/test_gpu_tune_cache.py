@@ -61,7 +61,7 @@ def test_tune_cache_reused_by_a_second_process(tmp_path, oracle64):
     d1 = _child(cache, str(tmp_path / "y1.npy"))
     assert "tuned:" in d1 and "cache file" not in d1, d1
     lines = open(cache).read().splitlines()
-    assert lines[0].startswith("#minfft-tune v4 arch=gfx950"), lines[0]
+    assert lines[0].startswith("#minfft-tune v5 arch=gfx950"), lines[0]
     assert any(ln.startswith("p1.k0.d1.16384x1x1.c6") for ln in lines[1:]), lines
     print("[case] tune cache: second process reuses it", flush=True)
     d2 = _child(cache, str(tmp_path / "y2.npy"))
@@ -78,7 +78,7 @@ def test_tune_cache_reused_by_a_second_process(tmp_path, oracle64):
         assert e <= TOL32, e
     print("[case] tune cache: a file of another build is ignored and replaced", flush=True)
     with open(cache, "w") as f:
-        f.write("#minfft-tune v4 arch=gfx950 cus=256 build=another\n")
+        f.write("#minfft-tune v5 arch=gfx950 cus=256 build=another\n")
         f.write("p1.k0.d1.16384x1x1.c6 1 1 1 1 1 1 1 1 1 1 1 1 0 1 5 64 1.0 1.0\n")
     d3 = _child(cache, str(tmp_path / "y3.npy"))
     assert "tuned:" in d3 and "cache file" not in d3, d3
