@@ -74,6 +74,10 @@ void register_big(BigTable& t) {
     t.mfma512nt[1] = reinterpret_cast<const void*>(&mfma512_kernel<true, true>);
     t.mfma512cfnt[0] = reinterpret_cast<const void*>(&mfma512_cf_kernel<false, true>);
     t.mfma512cfnt[1] = reinterpret_cast<const void*>(&mfma512_cf_kernel<true, true>);
+    t.mfma512cs[0][0] = reinterpret_cast<const void*>(&mfma512_cs_kernel<false, false>);
+    t.mfma512cs[0][1] = reinterpret_cast<const void*>(&mfma512_cs_kernel<true, false>);
+    t.mfma512cs[1][0] = reinterpret_cast<const void*>(&mfma512_cs_kernel<false, true>);
+    t.mfma512cs[1][1] = reinterpret_cast<const void*>(&mfma512_cs_kernel<true, true>);
     t.mfma2[0][0][0] = reinterpret_cast<const void*>(&mfma512_kernel<false, false, 2>);
     t.mfma2[0][0][1] = reinterpret_cast<const void*>(&mfma512_kernel<true, false, 2>);
     t.mfma2[0][1][0] = reinterpret_cast<const void*>(&mfma512_kernel<false, true, 2>);

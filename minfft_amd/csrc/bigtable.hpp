@@ -23,6 +23,8 @@ struct BigTable {
     const void* mfma512cfnt[2];
     // the same with both radix-16 stages on the matrix cores, [column][nt][inverse]
     const void* mfma2[2][2][2];
+    // radix-2-split column variant (32 transforms per work-group), [nt][inverse]
+    const void* mfma512cs[2][2];
     // single-pass radix-2-split kernels (split2.hpp), by log2 L; variant 1
     // runs the halves with 32 points per lane (radix-32 passes)
     const void* split2[2][17][2][2];  // [prec][log2 L][variant][inverse]

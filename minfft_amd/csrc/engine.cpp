@@ -445,6 +445,23 @@ static int launch_tile(Prec prec, int kind, int logL, const void* tw, const void
         const int pi = prec == PREC_F64 ? 0 : 1, iv = kind == K_INVDFT;
         const long long far_e = (long long)(1 << 20) / (long long)cs;
         const bool cs_on = tn.csplit > 0 || (tn.csplit < 0 && (prec == PREC_F64 || in_e >= far_e || out_e >= far_e));
+        // split-FP16: the matrix-core form of the same tile (mfma.hpp), 32
+        // transforms per work-group, in place of the 16-transform column variant
+        if (cs_on && s16cf && !in_row && !out_row && tn.mstages != 2 && bt.mfma512cs[0][iv]) {
+            const int C = 32;
+            const bool nt = ntr * (unsigned long long)L * cs >= kStreamBytes;
+            A.C = C;
+            A.lgC = 5;
+            A.sc = slot_len(L / 2);
+            const unsigned long long tiles = (ntr + C - 1) / C;
+            A.ntiles = (unsigned)tiles;
+            const bool far_stride = in_e >= far_e || out_e >= far_e;
+            A.xcd = (tn.xcd && !far_stride && tiles >= 2ull * D.nxcd) ? D.nxcd : 1;
+            A.cfast = 1;
+            A.wave = 0;
+            void* args[] = {&A};
+            return launch_fn(bt.mfma512cs[nt ? 1 : 0][iv], (unsigned)tiles, 512u, args, (size_t)C * A.sc * cs, st);
+        }
         if (cs_on && cplx && !split16 && !e32 && !e4 && fs_logN == 0 && fn == kt.tile[kind][logL] &&
             A.cfast && A.in_mode == IO_DIRECT && A.out_mode == IO_DIRECT && in_cf && out_cf && !in_row &&
             !out_row && bt.csplit[pi][logL][0][iv]) {

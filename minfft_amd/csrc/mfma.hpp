@@ -531,4 +531,106 @@ __global__ void __launch_bounds__(512) mfma512_cf_kernel(TileArgs A) {
     }
 }
 
+// Radix-2-split column variant (the split column tile of csplit.hpp with the
+// radix-16 stage of each half on the matrix cores): 32 adjacent transforms
+// per work-group (256-byte row segments of FP32 columns), two work-groups
+// per CU.  a[n] = x[n] + x[n+256], b[n] = (x[n] - x[n+256]) W_512^n in
+// registers; each 256-point half then runs its first radix-16 pass
+// (butterflies j < 16 over rows j + 16r, output 16j + k) as split-FP16
+// products and its second (radix 16, W_256) on the vector ALUs; X[2k] and
+// X[2k+1] go to neighbouring rows.  Wavefront w takes transforms
+// 16 (w & 1) + c and butterflies 4 (w >> 1) + i for the matrix-core pass, so
+// the two wavefronts of a pair read the two 128-byte halves of the same
+// rows; the vector pass and the stores use the c-fast mapping over all 32.
+template <bool INV, bool NT = false>
+__global__ void __launch_bounds__(512, 4) mfma512_cs_kernel(TileArgs A) {
+    using C2 = float2;
+    constexpr int H = 256, E = 16, TPT = H / E, CT = 32;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    C2* lds = reinterpret_cast<C2*>(smem_raw);
+    const C2* __restrict__ tw = reinterpret_cast<const C2*>(A.tw);
+
+    const int t = threadIdx.x;
+    const unsigned tile = tile_id(A);
+    const int w = t >> 6, g = (t >> 4) & 3;
+    const int cm = 16 * (w & 1) + (t & 15);  // matrix-core pass: transform
+    const int jb = 4 * (w >> 1);             // and its first butterfly
+    const int cc = t & (CT - 1), tl = t >> 5;  // vector pass: c-fast core
+    const TransformRef mm = locate(A, tile * (unsigned)CT + (unsigned)cm);
+    const TransformRef mc = locate(A, tile * (unsigned)CT + (unsigned)cc);
+    C2* slot_m = lds + (long long)cm * A.sc;
+    C2* slot_c = lds + (long long)cc * A.sc;
+
+    // inputs n = j + 16r (j = jb + i, r = 4g + q) and n + 256
+    C2 a[4][4], b[4][4];
+    if (mm.valid) {
+        const long long es = A.in_e;
+        const C2* __restrict__ p0 = reinterpret_cast<const C2*>(A.in) + mm.ib + (long long)(jb + 64 * g) * es;
+        const long long half = 256 * es;
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const long long o = (long long)(i + 16 * q) * es;
+                const C2 x0 = NT ? ld_nt<float>(&p0[o]) : p0[o];
+                const C2 x1 = NT ? ld_nt<float>(&p0[o + half]) : p0[o + half];
+                const C2 e = twl<float>(tw, (unsigned)(512 + jb + 64 * g + i + 16 * q));  // W_512^n
+                const C2 d = sub<float>(x0, x1);
+                a[i][q] = add<float>(x0, x1);
+                b[i][q] = INV ? cmulc<float>(d, e) : cmul<float>(d, e);
+            }
+    } else {
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) a[i][q] = b[i][q] = make_float2(0.f, 0.f);
+    }
+    C2* __restrict__ out = reinterpret_cast<C2*>(A.out) + mc.ob;
+    const long long os = A.out_e;
+#pragma unroll
+    for (int hf = 0; hf < 2; ++hf) {
+        C2 (&x)[4][4] = hf ? b : a;
+        // ---- pass 1 on the matrix cores; one scale per transform over the
+        // wavefront's four butterflies of it (lanes c + 16g).  The operands
+        // are loaded per half (an opaque lane index keeps the loads here):
+        // held across the vector pass they would push the second half's
+        // inputs out of the 128-register budget ----
+        {
+            const Mfma32Operands op = mfma32_operands<INV>(opaque(t & 63));
+            const float m1 = opaque_m1();
+            float m = 0.f;
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) m = fmaxf(m, fmaxf(fabsf(x[i][q].x), fabsf(x[i][q].y)));
+            m = fmaxf(m, __shfl_xor(m, 16));
+            m = fmaxf(m, __shfl_xor(m, 32));
+            const int sh = scale_exp(m);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int j = jb + i;
+                float2 y[4];
+                mfma16_columns(x[i], sh, op, m1, y);
+                C2* d = slot_m + (17 * j + 4 * g);  // lpad(16j + 4g + q)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) d[q] = y[q];
+            }
+        }
+        __syncthreads();
+        // ---- pass 2 (radix 16, W_256) on the vector ALUs ----
+        C2 v[E];
+        Core<float, 8, E, INV>::read_owned(v, slot_c, tl);
+        stockham_pass<float, H, E, 16, 16, INV, true>(v, slot_c, tl, tw);  // v: DFT at k = tl + m*TPT
+        if (mc.valid) {
+#pragma unroll
+            for (int m = 0; m < E; ++m) {
+                const long long row = 2LL * (tl + m * TPT) + hf;
+                if constexpr (NT) st_nt<float>(&out[row * os], v[m]);
+                else out[row * os] = v[m];
+            }
+        }
+        __syncthreads();  // the slot is read before the next half's pass 1 writes it
+    }
+}
+
 }  // namespace mfx

@@ -223,6 +223,54 @@ def test_split_fp16_mfma_batch(oracle64):
     print(f"split-FP16 worst rel-L2 {worst:.2e}")
 
 
+def test_split_fp16_split_column_tile(oracle64):
+    """The matrix-core form of the radix-2-split column tile (32 transforms
+    per work-group) on 512-point columns, pinned on (csplit=1), off and by
+    default: 2D and 3D entries, a partial last tile (3 x 16 columns), large
+    values (per-column scaling), forward and inverse, every entry against
+    the FP64 oracle at rel-L2 <= 1e-6; the three forms agree to 1e-6."""
+    torch = _torch()
+    shapes = [(512, 64), (512, 16), (512, 8, 16), (512, 32)]
+    rng = np.random.default_rng(33)
+    entries, xs = [], []
+    for i in range(10):
+        shape = shapes[i % len(shapes)]
+        kind = "dft" if i % 2 == 0 else "invdft"
+        x = make_input(kind, shape, 32, int(rng.integers(1 << 30)))
+        if i % 3 == 1:
+            x = (x * np.float32(3e4)).astype(np.complex64)
+        entries.append((kind, shape))
+        xs.append(x)
+    ioffs, ooffs, pos = [], [], 0
+    for x in xs:
+        ioffs.append(pos)
+        ooffs.append(pos)
+        pos += x.nbytes + 256
+    hin = np.zeros(pos, dtype=np.uint8)
+    for x, o in zip(xs, ioffs):
+        hin[o:o + x.nbytes] = np.frombuffer(x.tobytes(), dtype=np.uint8)
+    din = torch.from_numpy(hin).cuda()
+    b = mf.Batch([(k, sh, io, oo) for (k, sh), io, oo in zip(entries, ioffs, ooffs)], mf.PREC_SPLIT_FP16)
+    ngroups = b.describe().count("\n")
+    refs = [oracle64.run(k, x.astype(np.complex128), sh) for (k, sh), x in zip(entries, xs)]
+    outs = {}
+    for spec in ("csplit=1", "csplit=0", ""):
+        for gi in range(ngroups):
+            b.schedule(gi, spec)
+        dout = torch.zeros(pos, dtype=torch.uint8, device="cuda")
+        print(f"[case] split-FP16 column tile schedule '{spec}'", flush=True)
+        b.run(din.data_ptr(), dout.data_ptr(), None)
+        torch.cuda.synchronize()
+        h = dout.cpu().numpy()
+        got = [h[o:o + x.nbytes].view(np.complex64).reshape(sh) for (k, sh), x, o in zip(entries, xs, ooffs)]
+        for i, (y, r) in enumerate(zip(got, refs)):
+            assert rel_l2(y, r) <= TOL32, (spec, i, entries[i], rel_l2(y, r))
+        outs[spec] = got
+    b.close()
+    for i in range(len(entries)):
+        assert rel_l2(outs["csplit=1"][i], outs["csplit=0"][i]) <= 1e-6, i
+
+
 @pytest.mark.parametrize("prec", [32, 64])
 def test_persistent_short_lengths(oracle64, prec, monkeypatch):
     """Short 1D complex groups (2^4 .. 2^12, both directions) run in one
