@@ -271,6 +271,45 @@ def test_split_fp16_split_column_tile(oracle64):
         assert rel_l2(outs["csplit=1"][i], outs["csplit=0"][i]) <= 1e-6, i
 
 
+@pytest.mark.parametrize("mode", ["fp32", "split-fp16", "fp64"])
+def test_split_column_tiles_in_place(oracle64, mode):
+    """The radix-2-split column tiles (pinned on: csplit=1) run in place give
+    the same bits as out of place -- every work-group reads all of its
+    columns before it writes -- and match the oracle."""
+    torch = _torch()
+    prec = 64 if mode == "fp64" else 32
+    pm = {"fp32": mf.PREC_FP32, "split-fp16": mf.PREC_SPLIT_FP16, "fp64": mf.PREC_FP64}[mode]
+    isz = 16 if prec == 64 else 8
+    shapes = [("dft", (512, 64)), ("invdft", (512, 16)), ("dft", (1024, 32)), ("invdft", (512, 8, 16))] * 2
+    entries, off = [], 0
+    for kind, sh in shapes:
+        entries.append((kind, sh, off, off))
+        off += int(np.prod(sh)) * isz
+    rdt = torch.float64 if prec == 64 else torch.float32
+    g = torch.Generator(device="cuda")
+    g.manual_seed(9)
+    x = torch.complex(torch.rand(off // isz, generator=g, device="cuda", dtype=rdt) * 2 - 1,
+                      torch.rand(off // isz, generator=g, device="cuda", dtype=rdt) * 2 - 1)
+    b = mf.Batch(entries, pm)
+    for gi in range(b.describe().count("\n")):
+        b.schedule(gi, "csplit=1")
+    print(f"[case] split column tiles in place, {mode}", flush=True)
+    y = torch.zeros_like(x)
+    b.run(x.data_ptr(), y.data_ptr(), None)
+    z = x.clone()
+    b.run(z.data_ptr(), z.data_ptr(), None)
+    torch.cuda.synchronize()
+    b.close()
+    assert torch.equal(z, y)
+    xh, yh = x.cpu().numpy(), y.cpu().numpy()
+    for i in (0, 1, 2, 3):
+        kind, sh, o, _ = entries[i]
+        n = int(np.prod(sh))
+        ref = oracle64.run(kind, xh[o // isz:o // isz + n].astype(np.complex128).reshape(sh), sh)
+        e = rel_l2(yh[o // isz:o // isz + n].reshape(sh), ref)
+        assert e <= (tol64(sh) if prec == 64 else TOL32), (mode, i, kind, sh, e)
+
+
 @pytest.mark.parametrize("prec", [32, 64])
 def test_persistent_short_lengths(oracle64, prec, monkeypatch):
     """Short 1D complex groups (2^4 .. 2^12, both directions) run in one
