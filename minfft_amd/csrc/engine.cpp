@@ -436,15 +436,19 @@ static int launch_tile(Prec prec, int kind, int logL, const void* tw, const void
     // adjacent transforms with direct c-fast I/O; half of each transform in
     // the LDS at a time, so a work-group takes twice the columns of the
     // whole-transform tile (twice the row segment) in the same LDS.  By
-    // default for FP64 column passes and for FP32 ones with strides of
-    // 1 MiB and more; 512^3 per pass (kernel trace): FP32 outer axis (2 MiB)
-    // 473 -> 433 us, middle axis (4 KiB) 370 -> 390 us (so kept off there);
-    // FP64 outer 987 -> 847 us, middle 750 -> 723 us
+    // default for FP64 column passes and for FP32 512-point ones with
+    // strides of 1 MiB and more; 512^3 per pass (kernel trace): FP32 outer
+    // axis (2 MiB) 473 -> 433 us, middle axis (4 KiB) 370 -> 390 us (so kept
+    // off there); FP64 outer 987 -> 847 us, middle 750 -> 723 us
     {
         const BigTable& bt = bigtable();
         const int pi = prec == PREC_F64 ? 0 : 1, iv = kind == K_INVDFT;
         const long long far_e = (long long)(1 << 20) / (long long)cs;
-        const bool cs_on = tn.csplit > 0 || (tn.csplit < 0 && (prec == PREC_F64 || in_e >= far_e || out_e >= far_e));
+        // (FP32 1024-point columns keep the whole-transform tile by default:
+        // its 16 columns already span 128 bytes; 1024^3 outer axis 11.67-11.80
+        // against 11.82-11.89 ms per transform with the split tile)
+        const bool cs_on = tn.csplit > 0 ||
+                           (tn.csplit < 0 && (prec == PREC_F64 || (logL == 9 && (in_e >= far_e || out_e >= far_e))));
         // split-FP16: the matrix-core form of the same tile (mfma.hpp), 32
         // transforms per work-group, in place of the 16-transform column variant
         if (cs_on && s16cf && !in_row && !out_row && tn.mstages != 2 && bt.mfma512cs[0][iv]) {

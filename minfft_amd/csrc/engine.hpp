@@ -110,7 +110,7 @@ struct Tune {
     int csplit = -1;    // complex column passes (both sides columns of adjacent transforms)
                         // on the radix-2-split column tile (csplit.hpp) where instantiated:
                         // 1 = on, 0 = off (whole-transform tile), -1 = FP64 passes and FP32
-                        // ones with strides of 1 MiB and more (the measured best)
+                        // 512-point ones with strides of 1 MiB and more (the measured best)
 };
 struct Plan {
     unsigned magic = 0x6d667870u;
