@@ -310,14 +310,15 @@ def test_split_column_tiles_in_place(oracle64, mode):
         assert e <= (tol64(sh) if prec == 64 else TOL32), (mode, i, kind, sh, e)
 
 
-def test_dct2_columns_of_1024_points(oracle64):
-    """FP32 DCT-2 / DST-2 columns of 1024 points (the height of the DCT
-    image batch) match the oracle: partial last tiles (16 columns; 48
-    through an offset table), a 3D array, a two-column array, offset tables
-    with even and with odd item offsets; in-place runs give the bits of
-    out-of-place ones."""
+def test_dct_columns_of_1024_points(oracle64):
+    """FP32 DCT/DST columns of 1024 points (the height of the DCT image
+    batch), every kind, match the oracle: partial last tiles (16 columns;
+    48 through an offset table), a 3D array, two-column arrays, offset
+    tables with even and with odd item offsets; in-place runs give the bits
+    of out-of-place ones."""
     torch = _torch()
-    cases = [("dct2", (1024, 64)), ("dst2", (1024, 16)), ("dct2", (1024, 4, 8)), ("dst2", (1024, 2))]
+    cases = [("dct2", (1024, 64)), ("dst2", (1024, 16)), ("dct2", (1024, 4, 8)), ("dst2", (1024, 2)),
+             ("dct3", (1024, 32)), ("dst3", (1024, 2)), ("dct4", (1024, 16)), ("dst4", (1024, 4, 8))]
     entries, off = [], 0
     for kind, sh in cases:
         entries.append((kind, sh, off * 4, off * 4))
@@ -333,7 +334,7 @@ def test_dct2_columns_of_1024_points(oracle64):
     g.manual_seed(21)
     x = torch.rand(off, generator=g, device="cuda", dtype=torch.float32) * 2 - 1
     b = mf.Batch(entries, mf.PREC_FP32)
-    print("[case] dct2/dst2 columns of 1024 points", flush=True)
+    print("[case] dct/dst columns of 1024 points", flush=True)
     y = torch.zeros_like(x)
     b.run(x.data_ptr(), y.data_ptr(), None)
     z = x.clone()
