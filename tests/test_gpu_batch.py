@@ -310,8 +310,9 @@ def test_split_column_tiles_in_place(oracle64, mode):
         assert e <= (tol64(sh) if prec == 64 else TOL32), (mode, i, kind, sh, e)
 
 
-def test_dct_columns_of_1024_points(oracle64):
-    """FP32 DCT/DST columns of 1024 points (the height of the DCT image
+@pytest.mark.parametrize("prec", [32, 64])
+def test_dct_columns_of_1024_points(oracle64, prec):
+    """FP32 and FP64 DCT/DST columns of 1024 points (the height of the DCT image
     batch), every kind, match the oracle: partial last tiles (16 columns;
     48 through an offset table), a 3D array, two-column arrays, offset
     tables with even and with odd item offsets; in-place runs give the bits
@@ -320,21 +321,22 @@ def test_dct_columns_of_1024_points(oracle64):
     cases = [("dct2", (1024, 64)), ("dst2", (1024, 16)), ("dct2", (1024, 4, 8)), ("dst2", (1024, 2)),
              ("dct3", (1024, 32)), ("dst3", (1024, 2)), ("dct4", (1024, 16)), ("dst4", (1024, 4, 8))]
     entries, off = [], 0
+    isz = prec // 8
     for kind, sh in cases:
-        entries.append((kind, sh, off * 4, off * 4))
+        entries.append((kind, sh, off * isz, off * isz))
         off += int(np.prod(sh))
     # a group of three entries at even item offsets, one of two at odd ones
     for kind, sh, o0, cnt in (("dct2", (1024, 16), 0, 3), ("dst2", (1024, 32), 1, 2)):
         n = int(np.prod(sh))
         for _ in range(cnt):
             start = off + o0
-            entries.append((kind, sh, start * 4, start * 4))
+            entries.append((kind, sh, start * isz, start * isz))
             off = start + n + 2
     g = torch.Generator(device="cuda")
     g.manual_seed(21)
-    x = torch.rand(off, generator=g, device="cuda", dtype=torch.float32) * 2 - 1
-    b = mf.Batch(entries, mf.PREC_FP32)
-    print("[case] dct/dst columns of 1024 points", flush=True)
+    x = torch.rand(off, generator=g, device="cuda", dtype=torch.float64 if prec == 64 else torch.float32) * 2 - 1
+    b = mf.Batch(entries, mf.PREC_FP64 if prec == 64 else mf.PREC_FP32)
+    print(f"[case] dct/dst columns of 1024 points, FP{prec}", flush=True)
     y = torch.zeros_like(x)
     b.run(x.data_ptr(), y.data_ptr(), None)
     z = x.clone()
@@ -343,12 +345,12 @@ def test_dct_columns_of_1024_points(oracle64):
     b.close()
     xh, yh, zh = x.cpu().numpy(), y.cpu().numpy(), z.cpu().numpy()
     for i, (kind, sh, o, _) in enumerate(entries):
-        o //= 4
+        o //= isz
         n = int(np.prod(sh))
         assert np.array_equal(zh[o:o + n], yh[o:o + n]), (i, kind, sh)
         ref = oracle64.run(kind, xh[o:o + n].astype(np.float64).reshape(sh), sh)
         e = rel_l2(yh[o:o + n].reshape(sh), ref)
-        assert e <= TOL32, (i, kind, sh, e)
+        assert e <= (tol64(sh) if prec == 64 else TOL32), (prec, i, kind, sh, e)
 
 
 @pytest.mark.parametrize("prec", [32, 64])
